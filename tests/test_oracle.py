@@ -1,0 +1,411 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Each test checks the oracle against something it does not itself compute:
+the reference's stored fixture (the log-6 twiddle tree, poly.rs:451-595),
+Python's hashlib BLAKE2s, closed forms from the reference tests
+(fri.rs:141-192, accumulation.rs:31-101, field.rs:41-78), or brute-force
+evaluation of the explicit polynomial bases in tests/refmath.py.
+"""
+import hashlib
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+import refmath as R
+
+P = R.P
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+
+
+# ------------------------------------------------------------------ fields
+def test_m31_edges():
+    for a in (0, 1, 2, P - 1, 1 << 30, (1 << 30) + 7):
+        for b in (0, 1, P - 1, 1 << 30, 12345):
+            assert O.m31_mul(a, b) == a * b % P
+            assert O.m31_add(a, b) == (a + b) % P
+            assert O.m31_sub(a, b) == (a - b) % P
+        assert O.m31_neg(a) == (-a) % P  # canonical: neg(0) = 0
+        if a:
+            assert O.m31_mul(a, O.m31_inv(a)) == 1
+
+
+def test_extension_closed_forms():
+    i = (0, 1, 0, 0)
+    u = (0, 0, 1, 0)
+    assert O.qm31_mul(i, i) == (P - 1, 0, 0, 0)  # i^2 = -1
+    assert O.qm31_mul(u, u) == (2, 1, 0, 0)  # u^2 = 2 + i
+    rng = random.Random(1)
+    for _ in range(200):
+        a = tuple(rng.randrange(P) for _ in range(4))
+        b = tuple(rng.randrange(P) for _ in range(4))
+        assert O.qm31_mul(a, b) == R.qm_mul(a, b)
+        assert O.qm31_add(a, b) == R.qm_add(a, b)
+        assert O.qm31_sub(a, b) == R.qm_sub(a, b)
+        if a != (0, 0, 0, 0):
+            assert O.qm31_mul(a, O.qm31_inv(a)) == (1, 0, 0, 0)
+        e = rng.randrange(1 << 40)
+        assert O.qm31_pow(a, e) == R.qm_pow(a, e)
+    for _ in range(50):
+        c = (rng.randrange(P), rng.randrange(P))
+        ci = O.cm31_inv(c)
+        assert R.cm_mul(c, ci) == (1, 0)
+    # an element with a zero coordinate pair exercises neg(0)
+    a = (5, 0, 0, 7)
+    assert O.qm31_mul(a, O.qm31_inv(a)) == (1, 0, 0, 0)
+    assert all(v < P for v in O.qm31_inv(a))
+
+
+# ------------------------------------------------------------------ circle group
+def test_circle_group():
+    g = O.point_at_index(1)
+    assert g == R.G
+    assert (g[0] ** 2 + g[1] ** 2) % P == 1
+    assert O.point_at_index(1 << 30) == (P - 1, 0)
+    assert O.point_at_index(0) == (1, 0)
+    rng = random.Random(2)
+    for _ in range(50):
+        a, b = rng.randrange(1 << 31), rng.randrange(1 << 31)
+        assert R.pt_mul(O.point_at_index(a), O.point_at_index(b)) == O.point_at_index((a + b) % (1 << 31))
+        assert O.point_at_index(a) == R.point_at_index(a)
+
+
+def test_circle_domain_points():
+    hi, hs, hl = R.canonic_half_coset(5)
+    pts = R.circle_domain_points(hi, hs, hl)
+    for i, p in enumerate(pts):
+        assert O.circle_domain_at(hi, hs, hl, i) == p
+    assert len(set(pts)) == 32
+
+
+# ------------------------------------------------------------------ twiddles
+def test_twiddle_tree_matches_reference_fixture():
+    f = FIX["interpolate_from_fib"]
+    rc = f["root_coset"]
+    tw, itw = O.precompute_twiddles(rc["initial_index"], rc["step_size"], rc["log_size"])
+    assert tw.tolist() == f["twiddles"]
+    assert itw.tolist() == f["itwiddles"]
+    # the root coset is CanonicCoset(7).half_coset
+    assert (rc["initial_index"], rc["step_size"], rc["log_size"]) == R.canonic_half_coset(7)
+
+
+# ------------------------------------------------------------------ FFT
+def _brute_evals(coeffs, half_coset):
+    pts = R.circle_domain_points(*half_coset)
+    k = len(coeffs).bit_length() - 1
+    return [R.circle_poly_eval_m31(list(map(int, coeffs)), pts[R.brev(i, k)]) for i in range(len(coeffs))]
+
+
+def test_interpolate_fixture_domain_brute_force():
+    f = FIX["interpolate_from_fib"]
+    d = f["domain_half_coset"]
+    values = np.array(f["values"], np.uint32)
+    coeffs = O.interpolate(values, d["initial_index"], np.array(f["itwiddles"], np.uint32))
+    assert _brute_evals(coeffs, (d["initial_index"], d["step_size"], d["log_size"])) == f["values"]
+    back = O.evaluate(coeffs, d["initial_index"], np.array(f["twiddles"], np.uint32))
+    assert back.tolist() == f["values"]
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 4, 5, 6, 7])
+def test_fft_brute_force_canonic(log_n):
+    rng = np.random.default_rng(log_n)
+    hc = R.canonic_half_coset(log_n)
+    root = R.canonic_half_coset(log_n + 2)  # a larger tree serves smaller domains
+    tw, itw = O.precompute_twiddles(*root)
+    values = rng.integers(0, P, 1 << log_n, dtype=np.uint32)
+    coeffs = O.interpolate(values, hc[0], itw)
+    assert _brute_evals(coeffs, hc) == values.tolist()
+    assert O.evaluate(coeffs, hc[0], tw).tolist() == values.tolist()
+
+
+def test_interpolate_2_fixture_roundtrip():
+    # poly.rs:291-327 input (canonical order) -> bit-reversed circle evaluation
+    vals = np.array(FIX["interpolate_2_values"], np.uint32)
+    ev = O.new_canonical_ordered(vals)
+    hc = R.canonic_half_coset(5)
+    tw, itw = O.precompute_twiddles(*hc)
+    coeffs = O.interpolate(ev, hc[0], itw)
+    assert _brute_evals(coeffs, hc) == ev.tolist()
+
+
+def test_lde_brute_force():
+    rng = np.random.default_rng(9)
+    log_n, blow = 5, 1
+    tw, itw = O.precompute_twiddles(*R.canonic_half_coset(log_n + blow))
+    cols = [rng.integers(0, P, 1 << log_n, dtype=np.uint32) for _ in range(3)]
+    coeffs, ldes = O.lde_columns(cols, blow, tw, itw)
+    big = R.canonic_half_coset(log_n + blow)
+    pts = R.circle_domain_points(*big)
+    for c, co, lde in zip(cols, coeffs, ldes):
+        assert _brute_evals(co, R.canonic_half_coset(log_n)) == c.tolist()
+        m = log_n + blow
+        want = [R.circle_poly_eval_m31(co.tolist(), pts[R.brev(i, m)]) for i in range(1 << m)]
+        assert lde.tolist() == want
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 2, 3, 6])
+def test_eval_at_point_brute_force(log_n):
+    rng = random.Random(log_n)
+    coeffs = [rng.randrange(P) for _ in range(1 << log_n)]
+    px, py = R.random_qm31_point(rng)
+    assert O.eval_at_point(np.array(coeffs, np.uint32), px, py) == R.circle_poly_eval_qm31(coeffs, px, py)
+
+
+def test_new_canonical_ordered_semantics():
+    # canonical (coset-order) values land at the bit-reversed circle-domain slot of the same point
+    log_n = 5
+    n = 1 << log_n
+    coset = R.coset_points(1 << (30 - log_n), 1 << (31 - log_n), log_n)  # CanonicCoset(n).coset
+    dom = R.circle_domain_points(*R.canonic_half_coset(log_n))
+    vals = np.arange(100, 100 + n, dtype=np.uint32)
+    out = O.new_canonical_ordered(vals)
+    for k, p in enumerate(coset):
+        j = dom.index(p)
+        assert out[R.brev(j, log_n)] == vals[k]
+
+
+# ------------------------------------------------------------------ columns
+def test_bit_reverse():
+    a = np.arange(1 << 10, dtype=np.uint32)
+    b = O.bit_reverse(a)
+    assert all(b[i] == R.brev(i, 10) for i in range(1 << 10))
+    assert (O.bit_reverse(b) == a).all()
+    q = np.arange(4 * 256, dtype=np.uint32)
+    qb = O.bit_reverse_qm31(q).reshape(-1, 4)
+    for i in range(256):
+        assert qb[i].tolist() == q.reshape(-1, 4)[R.brev(i, 8)].tolist()
+
+
+def test_batch_inverse():
+    a = np.arange(1, 1 + (1 << 12), dtype=np.uint32)
+    inv = O.batch_inverse_m31(a)
+    assert all(int(x) * int(y) % P == 1 for x, y in zip(a, inv))
+    rng = np.random.default_rng(3)
+    q = rng.integers(1, P, 4 * 512, dtype=np.uint32)
+    qi = O.batch_inverse_qm31(q).reshape(-1, 4)
+    for x, y in zip(q.reshape(-1, 4), qi):
+        assert R.qm_mul(tuple(map(int, x)), tuple(map(int, y))) == (1, 0, 0, 0)
+
+
+# ------------------------------------------------------------------ blake2s / merkle
+@pytest.mark.parametrize("n", [0, 1, 3, 63, 64, 65, 128, 200])
+def test_blake2s256_vs_hashlib(n):
+    data = bytes((7 * i + 3) % 256 for i in range(n))
+    assert O.blake2s256(data) == hashlib.blake2s(data, digest_size=32).digest()
+
+
+def test_compress_parameters_vs_hashlib():
+    # one-block message: h = IV ^ param, counter = len, last-block flag all ones
+    iv = [0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19]
+    for n in (5, 40, 64):
+        data = bytes(range(n))
+        h = list(iv)
+        h[0] ^= 0x01010020
+        m = np.frombuffer(data + bytes(64 - n), dtype="<u4")
+        out = O.blake2s_compress(h, m, n, 0, 0xFFFFFFFF, 0)
+        assert out.astype("<u4").tobytes() == hashlib.blake2s(data, digest_size=32).digest()
+
+
+def _hash_node_model(children, values):
+    """stwo Blake2sMerkleHasher::hash_node restated in Python: zero state,
+    optional compression of left||right, then zero-padded 16-value chunks."""
+    st = np.zeros(8, np.uint32)
+    if children is not None:
+        st = O.blake2s_compress(st, np.concatenate(children))
+    vals = list(values) + [0] * ((-len(values)) % 16)
+    for c in range(0, len(vals), 16):
+        st = O.blake2s_compress(st, np.array(vals[c:c + 16], np.uint32))
+    return st
+
+
+def test_merkle_layers_reference_shapes():
+    # blake2s.rs:109-171 shapes: 35 columns at log 12 then 16 columns at log 11, value row*col
+    def cols(nc, n):
+        return [np.array([(r * c) % P for r in range(n)], np.uint32) for c in range(nc)]
+    c12, c11 = cols(35, 1 << 12), cols(16, 1 << 11)
+    l12 = O.merkle_commit_on_layer(12, None, c12).reshape(-1, 8)
+    l11 = O.merkle_commit_on_layer(11, l12.reshape(-1), c11).reshape(-1, 8)
+    for r in (0, 1, 77, 4095):
+        assert (l12[r] == _hash_node_model(None, [int(c[r]) for c in c12])).all()
+    for r in (0, 5, 2047):
+        assert (l11[r] == _hash_node_model((l12[2 * r], l12[2 * r + 1]), [int(c[r]) for c in c11])).all()
+    # a node layer without columns, and a leaf layer without columns (no compression: zero hash)
+    l10 = O.merkle_commit_on_layer(10, l11.reshape(-1), []).reshape(-1, 8)
+    assert (l10[3] == _hash_node_model((l11[6], l11[7]), [])).all()
+    assert (O.merkle_commit_on_layer(2, None, []) == 0).all()
+
+
+def test_merkle_commit_mixed_sizes_injection():
+    rng = np.random.default_rng(4)
+    sizes = [6, 4, 6, 3, 4]
+    cols = [rng.integers(0, P, 1 << s, dtype=np.uint32) for s in sizes]
+    layers = O.merkle_commit(cols)
+    # layer 4: children from layer 5 (no columns of size 5) plus the two log-4 columns in order
+    l5 = layers[5].reshape(-1, 8)
+    l4 = layers[4].reshape(-1, 8)
+    for r in range(16):
+        assert (l4[r] == _hash_node_model((l5[2 * r], l5[2 * r + 1]), [int(cols[1][r]), int(cols[4][r])])).all()
+    assert layers[0].size == 8
+
+
+# ------------------------------------------------------------------ FRI closed forms
+def test_fold_line_closed_form():
+    # fri.rs:141-192: poly = [even || odd] in the line basis, alpha = 19283,
+    # domain LineDomain(Coset::half_odds(3)); folded = 2 (f_e + alpha f_o) on domain.double()
+    even = [1, 2, 1, 3]
+    odd = [3, 5, 4, 1]
+    alpha = (19283, 0, 0, 0)
+    li, ls, ll = 1 << 26, 1 << 28, 3  # half_odds(3): initial G_{2^5}, step G_{2^3}
+    dom = R.coset_points(li, ls, ll)
+    values = [R.line_poly_eval(even + odd, R.qm(x)) for x, _ in dom]
+    values = [values[R.brev(i, ll)] for i in range(1 << ll)]
+    src = np.array(values, np.uint32).T
+    folded = O.fold_line(src, li, ls, alpha)
+    folded = [tuple(int(v) for v in folded[:, R.brev(i, ll - 1)]) for i in range(1 << (ll - 1))]
+    drp = R.coset_points(2 * li, 2 * ls, ll - 1)
+    for got, (x, _) in zip(folded, drp):
+        want = R.qm_scale(R.qm_add(R.line_poly_eval(even, R.qm(x)),
+                                   R.qm_mul(alpha, R.line_poly_eval(odd, R.qm(x)))), 2)
+        assert got == want
+
+
+def test_fold_circle_into_line_closed_form():
+    # a circle polynomial f = f0(x) + y f1(x) in the circle basis; folding p, -p with alpha
+    # gives dst' = dst alpha^2 + 2 (f0(x) + alpha f1(x)) at the half-coset x coordinates
+    rng = random.Random(5)
+    log_n = 5
+    n = 1 << log_n
+    coeffs = [tuple(rng.randrange(P) for _ in range(4)) for _ in range(n)]
+    hc = R.canonic_half_coset(log_n)
+    pts = R.circle_domain_points(*hc)
+    ev = [R.circle_poly_eval_qm31(coeffs, R.qm(x), R.qm(y)) for x, y in pts]
+    ev = [ev[R.brev(i, log_n)] for i in range(n)]
+    alpha = tuple(rng.randrange(P) for _ in range(4))
+    dst0 = [tuple(rng.randrange(P) for _ in range(4)) for _ in range(n // 2)]
+    out = O.fold_circle_into_line(np.array(dst0, np.uint32).T, np.array(ev, np.uint32).T, hc[0], hc[1], alpha)
+    f0 = [coeffs[2 * j] for j in range(n // 2)]
+    f1 = [coeffs[2 * j + 1] for j in range(n // 2)]
+
+    def circ_x_poly(cs, x):  # basis factors x, pi(x), ... for bits 0.. of the half index
+        k = len(cs).bit_length() - 1
+        fac, cur = [], R.qm(x)
+        for _ in range(k):
+            fac.append(cur)
+            cur = R.qm_sub(R.qm_scale(R.qm_mul(cur, cur), 2), R.qm(1))
+        tot = R.qm(0)
+        for idx, c in enumerate(cs):
+            t = c
+            for b in range(k):
+                if (idx >> b) & 1:
+                    t = R.qm_mul(t, fac[b])
+            tot = R.qm_add(tot, t)
+        return tot
+
+    half = R.coset_points(*hc)
+    a2 = R.qm_mul(alpha, alpha)
+    for i in range(n // 2):
+        x = half[R.brev(i, log_n - 1)][0]
+        want = R.qm_add(R.qm_mul(dst0[i], a2),
+                        R.qm_scale(R.qm_add(circ_x_poly(f0, x), R.qm_mul(alpha, circ_x_poly(f1, x))), 2))
+        assert tuple(int(v) for v in out[:, i]) == want
+
+
+# ------------------------------------------------------------------ quotients
+def _interp_low_degree(col4, log_size, half_initial, itw):
+    for k in range(4):
+        co = O.interpolate(col4[k], half_initial, itw)
+        if co[(1 << (log_size - 1)):].any():
+            return False
+    return True
+
+
+def test_quotients_low_degree_and_broken_sample():
+    rng = random.Random(6)
+    log_n, blow = 5, 1
+    m = log_n + blow
+    tw, itw = O.precompute_twiddles(*R.canonic_half_coset(m))
+    hc = R.canonic_half_coset(m)
+    nrng = np.random.default_rng(6)
+    trace = [nrng.integers(0, P, 1 << log_n, dtype=np.uint32) for _ in range(4)]
+    coeffs, ldes = O.lde_columns(trace, blow, tw, itw)
+    pts = [R.random_qm31_point(rng) for _ in range(3)]
+    batches = []
+    for b, p in enumerate(pts):
+        cols = [c for c in range(4) if (c + b) % 3 != 0]
+        batches.append((p, [(c, O.eval_at_point(coeffs[c], p[0], p[1])) for c in cols]))
+    rc = tuple(rng.randrange(P) for _ in range(4))
+    q = O.accumulate_quotients(hc[0], hc[1], m, ldes, rc, batches)
+    assert _interp_low_degree(q, m, hc[0], itw)
+    # one wrong sample value destroys low degree
+    bad = list(batches)
+    p, cv = bad[1]
+    cv = list(cv)
+    col, val = cv[0]
+    cv[0] = (col, ((val[0] + 1) % P,) + tuple(val[1:]))
+    bad[1] = (p, cv)
+    q2 = O.accumulate_quotients(hc[0], hc[1], m, ldes, rc, bad)
+    assert not _interp_low_degree(q2, m, hc[0], itw)
+
+
+def test_accumulate_closed_form():
+    # accumulation.rs:31-101: (1,2,3,4)+(5,6,7,8) and 1 + (P-1) = 0
+    n = 64
+    left = [np.full(n, v, np.uint32) for v in (1, 2, 3, 4)]
+    right = [np.full(n, v, np.uint32) for v in (5, 6, 7, 8)]
+    assert O.accumulate(left, right)[:, 0].tolist() == [6, 8, 10, 12]
+    ones = [np.full(n, 1, np.uint32)] * 4
+    pm1 = [np.full(n, P - 1, np.uint32)] * 4
+    assert not O.accumulate(ones, pm1).any()
+
+
+def test_wide_fib_quotient_low_degree():
+    # A trace with c_{i+2} = c_i^2 + c_{i+1}^2 has constraint quotients (over the trace
+    # coset's vanishing polynomial) of degree <= n/2.  Evaluated on a 4n domain (so the
+    # degree-n constraint is not aliased), their coefficients above index n vanish;
+    # a single corrupted trace cell breaks that.
+    log_n, n_cols = 4, 8
+    n = 1 << log_n
+    rng = np.random.default_rng(7)
+    t = [rng.integers(0, P, n, dtype=np.uint64), rng.integers(0, P, n, dtype=np.uint64)]
+    for i in range(2, n_cols):
+        t.append((t[i - 2] * t[i - 2] + t[i - 1] * t[i - 1]) % P)
+    trace = [c.astype(np.uint32) for c in t]
+    m = log_n + 2
+    tw, itw = O.precompute_twiddles(*R.canonic_half_coset(m))
+    hc = R.canonic_half_coset(m)
+    pts = R.circle_domain_points(*hc)
+
+    def vanish(p):
+        # CanonicCoset(log_n).coset = odd multiples of G_{2n}: x after log_n - 1 doublings
+        x = p[0]
+        for _ in range(log_n - 1):
+            x = (2 * x * x - 1) % P
+        return x
+    trace_coset = R.coset_points(1 << (30 - log_n), 1 << (31 - log_n), log_n)
+    assert all(vanish(p) == 0 for p in trace_coset)
+    den = np.array([vanish(pts[R.brev(i, m)]) for i in range(1 << m)], np.uint32)
+    assert den.all()
+    dinv = O.batch_inverse_m31(den)
+    alpha = (3, 1, 4, 1)
+    powers = np.array([R.qm_pow(alpha, k) for k in range(n_cols - 2)], np.uint32)
+
+    def quotient_degree_ok(tr):
+        _, ext = O.lde_columns(tr, 2, tw, itw)
+        out = O.wide_fib_constraint_quotients(ext, powers, dinv)
+        return all(not O.interpolate(out[k], hc[0], itw)[n + 1:].any() for k in range(4))
+    assert quotient_degree_ok(trace)
+    bad = [c.copy() for c in trace]
+    bad[5][3] = (int(bad[5][3]) + 1) % P
+    assert not quotient_degree_ok(bad)
+
+
+# ------------------------------------------------------------------ inputs
+def test_uniform_generator():
+    a = O.fill_uniform_m31(1 << 16, 0x5EED0002)
+    assert a.max() < P
+    assert abs(a.mean() / P - 0.5) < 0.01
+    assert (O.fill_uniform_m31(100, 0x5EED0002, offset=50) == a[50:150]).all()
+    assert O.lib().or_uniform_m31(0x5EED0002, 7) == a[7]
