@@ -1,0 +1,262 @@
+// Blake2s Merkle commitment: MerkleOps<Blake2sMerkleHasher>::commit_on_layer
+// (stwo_gpu_backend/src/blake2s.rs:19-97) and the whole-tree commit that stwo's
+// MerkleProver::commit drives one layer at a time.
+//
+// Hash of a node (cuda/src/blake2s.cu:97-151): state = 0^8; if the node has
+// children, state = F(state, left || right); then for each chunk of 16 columns
+// of this layer (zero-padded remainder) state = F(state, chunk).  F is the
+// BLAKE2s compression with the IV in v[8..15] and zero counter and flags
+// (blake2s.cu:56-95).
+//
+// MI355X design: one lane per node; a wavefront reads 64 consecutive rows of a
+// column (256-byte coalesced loads); the whole tree below the leaf layer is
+// built by a subtree kernel that produces up to 11 layers per launch in LDS,
+// injecting the columns of any layer (mixed-degree commitments) as it goes.
+#include <algorithm>
+
+#include "field.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace merkle {
+
+__device__ __forceinline__ uint32_t rotr(uint32_t x, uint32_t c) { return __builtin_amdgcn_alignbit(x, x, c); }
+
+#define B2_G(a, b, c, d, x, y) \
+  a = a + b + (x);             \
+  d = rotr(d ^ a, 16);         \
+  c = c + d;                   \
+  b = rotr(b ^ c, 12);         \
+  a = a + b + (y);             \
+  d = rotr(d ^ a, 8);          \
+  c = c + d;                   \
+  b = rotr(b ^ c, 7);
+
+#define B2_ROUND(s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, s10, s11, s12, s13, s14, s15) \
+  B2_G(v0, v4, v8, v12, m[s0], m[s1]);                                                  \
+  B2_G(v1, v5, v9, v13, m[s2], m[s3]);                                                  \
+  B2_G(v2, v6, v10, v14, m[s4], m[s5]);                                                 \
+  B2_G(v3, v7, v11, v15, m[s6], m[s7]);                                                 \
+  B2_G(v0, v5, v10, v15, m[s8], m[s9]);                                                 \
+  B2_G(v1, v6, v11, v12, m[s10], m[s11]);                                               \
+  B2_G(v2, v7, v8, v13, m[s12], m[s13]);                                                \
+  B2_G(v3, v4, v9, v14, m[s14], m[s15]);
+
+// h <- F(h, m) with zero counter and flags
+__device__ __forceinline__ void compress(uint32_t (&h)[8], const uint32_t (&m)[16]) {
+  uint32_t v0 = h[0], v1 = h[1], v2 = h[2], v3 = h[3], v4 = h[4], v5 = h[5], v6 = h[6], v7 = h[7];
+  uint32_t v8 = 0x6A09E667u, v9 = 0xBB67AE85u, v10 = 0x3C6EF372u, v11 = 0xA54FF53Au;
+  uint32_t v12 = 0x510E527Fu, v13 = 0x9B05688Cu, v14 = 0x1F83D9ABu, v15 = 0x5BE0CD19u;
+  B2_ROUND(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)
+  B2_ROUND(14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3)
+  B2_ROUND(11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4)
+  B2_ROUND(7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8)
+  B2_ROUND(9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13)
+  B2_ROUND(2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9)
+  B2_ROUND(12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11)
+  B2_ROUND(13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10)
+  B2_ROUND(6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5)
+  B2_ROUND(10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0)
+  h[0] ^= v0 ^ v8;
+  h[1] ^= v1 ^ v9;
+  h[2] ^= v2 ^ v10;
+  h[3] ^= v3 ^ v11;
+  h[4] ^= v4 ^ v12;
+  h[5] ^= v5 ^ v13;
+  h[6] ^= v6 ^ v14;
+  h[7] ^= v7 ^ v15;
+}
+
+// Absorb the columns of one layer at `row` (16-column chunks, zero-padded remainder).
+__device__ __forceinline__ void absorb_columns(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
+                                               uint32_t row) {
+  uint32_t c = 0;
+  for (; c + 16 <= n_cols; c += 16) {
+    uint32_t m[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) m[j] = cols[c + j][row];
+    compress(h, m);
+  }
+  if (c < n_cols) {
+    uint32_t m[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? cols[c + j][row] : 0u;
+    compress(h, m);
+  }
+}
+
+struct Hash {
+  uint32_t s[8];
+};
+
+__device__ __forceinline__ void store_hash(Hash* out, uint32_t i, const uint32_t (&h)[8]) {
+  uint4* o = reinterpret_cast<uint4*>(out + i);
+  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+__device__ __forceinline__ void load_children(const Hash* prev, uint32_t i, uint32_t (&m)[16]) {
+  const uint4* p = reinterpret_cast<const uint4*>(prev + 2 * (size_t)i);
+  uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+  m[0] = a.x; m[1] = a.y; m[2] = a.z; m[3] = a.w;
+  m[4] = b.x; m[5] = b.y; m[6] = b.z; m[7] = b.w;
+  m[8] = c.x; m[9] = c.y; m[10] = c.z; m[11] = c.w;
+  m[12] = d.x; m[13] = d.y; m[14] = d.z; m[15] = d.w;
+}
+
+// One layer, one lane per node (commit_on_first_layer / commit_on_layer_with_previous).
+__global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_cols, const uint32_t* const* cols,
+                                                    const Hash* prev, Hash* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= size) return;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (prev) {
+    uint32_t m[16];
+    load_children(prev, i, m);
+    compress(h, m);
+  }
+  absorb_columns(h, cols, n_cols, i);
+  store_hash(out, i, h);
+}
+
+// Columns injected into a layer: device array of column pointers.
+struct LayerCols {
+  const uint32_t* const* cols;
+  uint32_t n_cols;
+  uint32_t pad;
+};
+
+// Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b
+// reads 2^depth children hashes of layer top+1 (prev) and produces 2^(depth-1)
+// nodes of layer `top` ... down to one node of layer top-depth+1.  outs[k] and
+// inj[k] are indexed by layer k.
+constexpr int SUB_MAX_DEPTH = 11;  // 1024 nodes in the first produced layer
+__global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth,
+                                                      Hash* const* outs, const LayerCols* inj) {
+  __shared__ Hash s[1 << (SUB_MAX_DEPTH - 1)];
+  const uint32_t nodes0 = 1u << (depth - 1);
+  const uint32_t base0 = blockIdx.x * nodes0;
+  for (uint32_t d = 0; d < depth; d++) {
+    const uint32_t layer = top - d;
+    const uint32_t cnt = nodes0 >> d;
+    const uint32_t base = base0 >> d;
+    const LayerCols lc = inj[layer];
+    uint32_t h_keep[4][8];
+    uint32_t n_mine = 0;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x, n_mine++) {
+      uint32_t m[16];
+      if (d == 0) {
+        load_children(prev, base + i, m);
+      } else {
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+          m[w] = s[2 * i].s[w];
+          m[8 + w] = s[2 * i + 1].s[w];
+        }
+      }
+      uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      compress(h, m);
+      absorb_columns(h, lc.cols, lc.n_cols, base + i);
+      store_hash(outs[layer], base + i, h);
+#pragma unroll
+      for (int w = 0; w < 8; w++) h_keep[n_mine & 3][w] = h[w];
+    }
+    __syncthreads();  // all reads of s for this layer are done
+    n_mine = 0;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x, n_mine++) {
+#pragma unroll
+      for (int w = 0; w < 8; w++) s[i].s[w] = h_keep[n_mine & 3][w];
+    }
+    __syncthreads();
+  }
+}
+
+static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+// Whole tree.  cols/logs are host arrays; layers[k] (host array of device buffers) gets 2^k hashes.
+void commit_tree(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+                 uint32_t max_log) {
+  require(max_log <= 31, "merkle: log size too large");
+  // stable sort by decreasing size (MerkleProver::commit)
+  std::vector<std::vector<const uint32_t*>> by_layer(max_log + 1);
+  for (uint32_t i = 0; i < n_cols; i++) {
+    require(logs[i] <= max_log, "merkle: column larger than the tree");
+    by_layer[logs[i]].push_back(cols[i]);
+  }
+  // device tables: per-layer column pointers and LayerCols descriptors
+  std::vector<const uint32_t*> flat;
+  std::vector<size_t> off(max_log + 1);
+  for (uint32_t k = 0; k <= max_log; k++) {
+    off[k] = flat.size();
+    flat.insert(flat.end(), by_layer[k].begin(), by_layer[k].end());
+  }
+  if (flat.empty()) flat.push_back(nullptr);
+  DevBuf dcols = upload(flat.data(), flat.size() * sizeof(void*));
+  std::vector<LayerCols> lcs(max_log + 1);
+  for (uint32_t k = 0; k <= max_log; k++)
+    lcs[k] = {dcols.as<const uint32_t*>() + off[k], (uint32_t)by_layer[k].size(), 0};
+  DevBuf dlcs = upload(lcs.data(), lcs.size() * sizeof(LayerCols));
+  DevBuf douts = upload(layers, (max_log + 1) * sizeof(Hash*));
+  hipStream_t st = ctx().stream;
+
+  // leaf layer
+  timed("merkle_leaf", [&] {
+    layer_kernel<<<blocks_for(1ull << max_log, 256), 256, 0, st>>>(1u << max_log, lcs[max_log].n_cols,
+                                                                      lcs[max_log].cols, nullptr, layers[max_log]);
+  });
+  // node layers in subtree launches of up to SUB_MAX_DEPTH layers
+  int32_t top = (int32_t)max_log - 1;
+  while (top >= 0) {
+    uint32_t depth = std::min<uint32_t>(SUB_MAX_DEPTH, (uint32_t)top + 1);
+    uint32_t blocks = 1u << ((uint32_t)top + 1 - depth);
+    timed("merkle_subtree", [&] {
+      subtree_kernel<<<blocks, 256, 0, st>>>(layers[top + 1], (uint32_t)top, depth, douts.as<Hash* const>(),
+                                             dlcs.as<const LayerCols>());
+    });
+    top -= (int32_t)depth;
+  }
+}
+
+}  // namespace merkle
+}  // namespace sh
+
+using namespace sh;
+using sh::merkle::Hash;
+
+extern "C" void commit_on_first_layer(size_t size, size_t amount_of_columns, uint32_t* const* columns,
+                                      StwoBlake2sHash* result) {
+  guarded([&] {
+    require(size <= 0xffffffffull, "commit_on_first_layer: size too large");
+    if (!size) return;
+    timed("merkle_leaf", [&] {
+      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().stream>>>(
+          (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, nullptr, (Hash*)result);
+    });
+    sync();
+  });
+}
+
+extern "C" void commit_on_layer_with_previous(size_t size, size_t amount_of_columns, uint32_t* const* columns,
+                                              const StwoBlake2sHash* previous_layer, StwoBlake2sHash* result) {
+  guarded([&] {
+    require(size <= 0xffffffffull, "commit_on_layer_with_previous: size too large");
+    if (!size) return;
+    timed("merkle_node", [&] {
+      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().stream>>>(
+          (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, (const Hash*)previous_layer,
+          (Hash*)result);
+    });
+    sync();
+  });
+}
+
+extern "C" int stwo_hip_merkle_commit_async(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* log_sizes,
+                                            StwoBlake2sHash* const* layers, uint32_t max_log) {
+  return guarded([&] { merkle::commit_tree(n_cols, cols, log_sizes, (Hash* const*)layers, max_log); });
+}
+
+extern "C" int stwo_hip_merkle_root(const StwoBlake2sHash* root_layer, StwoBlake2sHash* root_out) {
+  return guarded([&] {
+    hip_check(hipMemcpyAsync(root_out, root_layer, 32, hipMemcpyDefault, ctx().stream), "hipMemcpy");
+    sync();
+  });
+}

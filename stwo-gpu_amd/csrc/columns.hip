@@ -1,0 +1,131 @@
+// Column operations: ColumnOps::bit_reverse_column (stwo_gpu_backend/src/column.rs:10-34),
+// PolyOps::new_canonical_ordered (poly.rs:23-33) and AccumulationOps::accumulate
+// (accumulation.rs:9-19).
+//
+// bit_reverse: the reference swaps a[i] <-> a[brev(i)] one element per thread
+// (cuda/src/bit_reverse.cu:6-16), which scatters every access.  Here an index
+// is split into [hi a bits | mid | lo a bits]; brev maps (hi, mid, lo) to
+// (brev lo, brev mid, brev hi), so the 2^a x 2^a tile at `mid` and the tile at
+// brev(mid) exchange places as a bit-reversed transpose through LDS, with
+// 2^a-element contiguous runs on both the read and the write side.
+#include "field.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace cols {
+
+constexpr int TA = 5;  // tile = 32 x 32 elements
+
+template <class T>
+__global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* a, uint32_t log_n) {
+  __shared__ T ta[1 << (2 * TA)];
+  __shared__ T tb[1 << (2 * TA)];
+  const uint32_t mid_bits = log_n - 2 * TA;
+  const uint32_t m = blockIdx.x;
+  const uint32_t mr = brev(m, mid_bits);
+  if (mr < m) return;  // the pair is handled by block mr
+  const uint32_t W = 1u << TA;
+  auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
+    return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
+  };
+  for (uint32_t e = threadIdx.x; e < W * W; e += blockDim.x) {
+    uint32_t hi = e >> TA, lo = e & (W - 1);
+    ta[e] = a[at(hi, m, lo)];
+    if (mr != m) tb[e] = a[at(hi, mr, lo)];
+  }
+  __syncthreads();
+  // element (hi, m, lo) goes to (brev lo, mr, brev hi)
+  for (uint32_t e = threadIdx.x; e < W * W; e += blockDim.x) {
+    uint32_t hi2 = e >> TA, lo2 = e & (W - 1);
+    uint32_t src = (brev(lo2, TA) << TA) | brev(hi2, TA);
+    a[at(hi2, mr, lo2)] = ta[src];
+    if (mr != m) a[at(hi2, m, lo2)] = tb[src];
+  }
+}
+
+template <class T>
+__global__ void bitrev_simple_kernel(T* a, uint32_t n, uint32_t log_n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = brev(i, log_n);
+  if (j > i) {
+    T t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+  }
+}
+
+template <class T>
+void bit_reverse(T* a, uint64_t n) {
+  require(is_pow2(n) && n <= (1ull << 31), "bit_reverse: size must be a power of two <= 2^31");
+  uint32_t log_n = log2_exact(n);
+  if (log_n >= 2 * TA) {
+    timed("bit_reverse", [&] {
+      bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), 256, 0, ctx().stream>>>(a, log_n);
+    });
+  } else if (n > 1) {
+    timed("bit_reverse", [&] {
+      bitrev_simple_kernel<T><<<ceil_div(n, 256), 256, 0, ctx().stream>>>(a, (uint32_t)n, log_n);
+    });
+  }
+}
+
+// canonic order -> circle-domain order: evens, then odds reversed (cuda/src/poly/twiddles.cu:4-15)
+__global__ void sort_values_kernel(const uint32_t* from, uint32_t* dst, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t half = n >> 1;
+  dst[i] = i < half ? from[2 * i] : from[n - 1 - 2 * (i - half)];
+}
+
+struct Soa4 {
+  uint32_t* c[4];
+};
+__global__ void accumulate_kernel(uint32_t n, Soa4 l, Soa4 r) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int k = 0; k < 4; k++) l.c[k][i] = add(l.c[k][i], r.c[k][i]);
+}
+
+}  // namespace cols
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" void bit_reverse_base_field(uint32_t* array, size_t size) {
+  guarded([&] {
+    cols::bit_reverse(array, size);
+    sync();
+  });
+}
+extern "C" void bit_reverse_secure_field(uint32_t* array, size_t size) {
+  guarded([&] {
+    cols::bit_reverse(reinterpret_cast<uint4*>(array), size);
+    sync();
+  });
+}
+extern "C" uint32_t* sort_values_and_permute_with_bit_reverse_order(const uint32_t* from, size_t size) {
+  uint32_t* dst = nullptr;
+  guarded([&] {
+    require(is_pow2(size) && size <= (1ull << 31), "sort_values: size must be a power of two");
+    dst = (uint32_t*)ctx().pool.alloc(size * 4);
+    timed("sort_values", [&] {
+      cols::sort_values_kernel<<<ceil_div(size, 256), 256, 0, ctx().stream>>>(from, dst, (uint32_t)size);
+    });
+    cols::bit_reverse(dst, size);
+    sync();
+  });
+  return dst;
+}
+extern "C" void accumulate(uint32_t size, uint32_t* const* left_columns, uint32_t* const* right_columns) {
+  guarded([&] {
+    if (!size) return;
+    cols::Soa4 l{{left_columns[0], left_columns[1], left_columns[2], left_columns[3]}};
+    cols::Soa4 r{{right_columns[0], right_columns[1], right_columns[2], right_columns[3]}};
+    timed("accumulate", [&] {
+      cols::accumulate_kernel<<<ceil_div(size, 256), 256, 0, ctx().stream>>>(size, l, r);
+    });
+    sync();
+  });
+}

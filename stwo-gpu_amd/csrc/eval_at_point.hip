@@ -1,0 +1,127 @@
+// PolyOps::eval_at_point (stwo_gpu_backend/src/poly.rs:84-94).
+//
+// For coefficients c[0 .. 2^L) and a QM31 point (x, y), the factors are
+// f = [pi^(L-2)(x), ..., pi(x), x, y] with pi(x) = 2x^2 - 1
+// (cuda/src/poly/eval_at_point.cu:99-110), and the value is the fold
+//   fold(c, f) = fold(lo half, f[1:]) + fold(hi half, f[1:]) * f[0],
+// i.e. bit b of a coefficient index selects factor f[L-1-b].
+//
+// Two launches: blocks reduce 2^11 coefficients each in LDS (bits 0..10), then
+// one block reduces the block results (remaining bits).  The reference's shared
+// memory aliasing (eval_at_point.cu:17-18) has no counterpart here: separate
+// arrays, and every level reads into registers before a barrier and writes after.
+#include <utility>
+#include <vector>
+
+#include "field.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace eap {
+
+constexpr int CB = 11;  // coefficients per first-pass block = 2^CB
+constexpr int NT = 256;
+
+struct Factors {
+  qm31 f[31];
+};
+
+// Reduce `cnt` level values in s (cnt power of two) with factors f[fi], f[fi-1], ...
+__device__ void tree_reduce(qm31* s, uint32_t cnt, const Factors& F, int fi) {
+  while (cnt > 1) {
+    uint32_t half = cnt >> 1;
+    qm31 r[4];
+    uint32_t k = 0;
+    for (uint32_t i = threadIdx.x; i < half; i += NT, k++) r[k & 3] = qadd(s[2 * i], qmul(s[2 * i + 1], F.f[fi]));
+    __syncthreads();
+    k = 0;
+    for (uint32_t i = threadIdx.x; i < half; i += NT, k++) s[i] = r[k & 3];
+    __syncthreads();
+    cnt = half;
+    fi--;
+  }
+}
+
+// First pass: block b folds coefficients [b 2^cb, (b+1) 2^cb) with f[L-1], ..., f[L-cb].
+__global__ __launch_bounds__(NT) void first_pass(const uint32_t* __restrict__ c, uint32_t L, uint32_t cb, Factors F,
+                                                 qm31* partial) {
+  __shared__ qm31 s[1 << (CB - 1)];
+  const uint32_t pairs = 1u << (cb - 1);
+  const size_t base = (size_t)blockIdx.x << cb;
+  const qm31 fy = F.f[L - 1];
+  for (uint32_t i = threadIdx.x; i < pairs; i += NT) {
+    uint32_t c0 = c[base + 2 * i], c1 = c[base + 2 * i + 1];
+    qm31 v = qscale(fy, c1);
+    v.x.a = add(v.x.a, c0);
+    s[i] = v;
+  }
+  __syncthreads();
+  tree_reduce(s, pairs, F, (int)L - 2);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// Later passes: block b folds partials [b 2^g, (b+1) 2^g) with f[fi], f[fi-1], ...
+__global__ __launch_bounds__(NT) void reduce_pass(const qm31* in, uint32_t g, Factors F, int fi, qm31* out) {
+  __shared__ qm31 s[1 << (CB - 1)];
+  const uint32_t cnt = 1u << g;
+  for (uint32_t i = threadIdx.x; i < cnt; i += NT) s[i] = in[((size_t)blockIdx.x << g) + i];
+  __syncthreads();
+  tree_reduce(s, cnt, F, fi);
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+}  // namespace eap
+
+qm31 eval_at_point_sync(const uint32_t* coeffs, uint64_t n, qm31 px, qm31 py) {
+  require(is_pow2(n) && n <= (1ull << 31), "eval_at_point: size must be a power of two");
+  uint32_t L = log2_exact(n);
+  if (L == 0) {
+    uint32_t c0;
+    hip_check(hipMemcpyAsync(&c0, coeffs, 4, hipMemcpyDeviceToHost, ctx().stream), "hipMemcpy");
+    sync();
+    return qm31{{c0, 0}, {0, 0}};
+  }
+  eap::Factors F{};
+  F.f[L - 1] = py;
+  qm31 x = px;
+  for (uint32_t i = 2; i <= L; i++) {
+    F.f[L - i] = x;
+    x = qsub(qadd(qsqr(x), qsqr(x)), qone());
+  }
+  uint32_t cb = std::min<uint32_t>(L, eap::CB);
+  uint32_t blocks = 1u << (L - cb);
+  DevBuf pa(sizeof(qm31) * blocks), pb(sizeof(qm31) * blocks);
+  timed("eval_at_point", [&] {
+    eap::first_pass<<<blocks, eap::NT, 0, ctx().stream>>>(coeffs, L, cb, F, pa.as<qm31>());
+  });
+  int fi = (int)(L - cb) - 1;
+  while (blocks > 1) {
+    uint32_t lb = log2_exact(blocks);
+    uint32_t g = std::min<uint32_t>(lb, eap::CB - 1);
+    uint32_t nb = blocks >> g;
+    timed("eval_at_point", [&] {
+      eap::reduce_pass<<<nb, eap::NT, 0, ctx().stream>>>(pa.as<qm31>(), g, F, fi, pb.as<qm31>());
+    });
+    std::swap(pa.p, pb.p);
+    fi -= (int)g;
+    blocks = nb;
+  }
+  qm31 r;
+  hip_check(hipMemcpyAsync(&r, pa.p, sizeof(qm31), hipMemcpyDeviceToHost, ctx().stream), "hipMemcpy");
+  sync();
+  return r;
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" StwoQM31 eval_at_point(const uint32_t* coeffs, uint32_t coeffs_size, StwoQM31 point_x, StwoQM31 point_y) {
+  StwoQM31 out{0, 0, 0, 0};
+  guarded([&] {
+    qm31 r = eval_at_point_sync(coeffs, coeffs_size, qm31{{point_x.a, point_x.b}, {point_x.c, point_x.d}},
+                                qm31{{point_y.a, point_y.b}, {point_y.c, point_y.d}});
+    out = StwoQM31{r.x.a, r.x.b, r.y.a, r.y.b};
+  });
+  return out;
+}

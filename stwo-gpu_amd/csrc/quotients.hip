@@ -1,0 +1,168 @@
+// DEEP quotients: QuotientOps::accumulate_quotients (stwo_gpu_backend/src/quotient.rs:17-115).
+//
+// For row r of the (bit-reversed) evaluation domain, p = domain.at(brev(r)), and
+// for each sample batch b (point P_b, samples (column j, value v_j)) in order
+// (cuda/src/quotients.cu:111-178):
+//   acc = acc * rc^|b| + (sum_j c_j col_j[r] - (a_j p.y + b_j)) * den_b(p)^-1
+// with line coefficients a = conj(v) - v, c = conj(P.y) - P.y, b = v c - a P.y,
+// each times alpha_j = rc^(j+1) (quotients.cu:49-88), and
+//   den_b(p) = (Re P.x - p.x) Im P.y - (Re P.y - p.y) Im P.x   in CM31 (quotients.cu:91-109).
+// The line coefficients use the cumulative offset of quotients.cu:147-168 (the
+// reference's quotients.cu:74 offset is wrong for batches of different sizes).
+//
+// MI355X design: the per-batch sums of a_j and b_j and the factor c (shared by
+// every sample of a batch, since c depends only on P) are folded on the host, so
+// a row costs one QM31 x M31 multiply-accumulate per sample, done on 64-bit
+// lazy sums (v_mad_u64_u32) and reduced once per batch.  Sample tables are
+// uniform across the wavefront and are read with scalar loads.
+#include <vector>
+
+#include "field.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace quot {
+
+struct Sample {
+  uint32_t col;
+  uint32_t pad;
+  qm31 alpha;  // rc^(j+1)
+};
+struct Batch {
+  uint32_t first, count;
+  qm31 c;      // conj(P.y) - P.y
+  qm31 a_sum;  // sum_j alpha_j (conj(v_j) - v_j)
+  qm31 b_sum;  // sum_j alpha_j (v_j c - (conj(v_j) - v_j) P.y)
+  qm31 coeff;  // rc^count
+  cm31 prx, pry, pix, piy;
+};
+
+__device__ __forceinline__ uint64_t fold64(uint64_t s) { return (s & P) + (s >> 31); }
+
+__global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
+                                                         const uint32_t* const* __restrict__ cols,
+                                                         const Sample* __restrict__ samples,
+                                                         const Batch* __restrict__ batches, uint32_t n_batches,
+                                                         uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
+                                                         uint32_t* __restrict__ o2, uint32_t* __restrict__ o3) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (1u << log_size)) return;
+  pt p = point_at_index(circle_domain_index(hi, hs, log_size - 1, brev(row, log_size)));
+  qm31 acc = qzero();
+  for (uint32_t b = 0; b < n_batches; b++) {
+    const Batch B = batches[b];
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint32_t j = 0;
+    for (; j + 2 <= B.count; j += 2) {
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const Sample S = samples[B.first + j + u];
+        uint64_t v = cols[S.col][row];
+        s0 += S.alpha.x.a * v;
+        s1 += S.alpha.x.b * v;
+        s2 += S.alpha.y.a * v;
+        s3 += S.alpha.y.b * v;
+      }
+      s0 = fold64(s0);
+      s1 = fold64(s1);
+      s2 = fold64(s2);
+      s3 = fold64(s3);
+    }
+    if (j < B.count) {
+      const Sample S = samples[B.first + j];
+      uint64_t v = cols[S.col][row];
+      s0 += S.alpha.x.a * v;
+      s1 += S.alpha.x.b * v;
+      s2 += S.alpha.y.a * v;
+      s3 += S.alpha.y.b * v;
+    }
+    qm31 sum = {{reduce63(s0), reduce63(s1)}, {reduce63(s2), reduce63(s3)}};
+    qm31 num = qsub(qmul(B.c, sum), qadd(qscale(B.a_sum, p.y), B.b_sum));
+    cm31 d = csub(cmul(cm31{sub(B.prx.a, p.x), B.prx.b}, B.piy), cmul(cm31{sub(B.pry.a, p.y), B.pry.b}, B.pix));
+    acc = qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
+  }
+  o0[row] = acc.x.a;
+  o1[row] = acc.x.b;
+  o2[row] = acc.y.a;
+  o3[row] = acc.y.b;
+}
+
+}  // namespace quot
+
+// points: n_batches x StwoCirclePointQM31 (host); indexes/values flattened in batch order (host).
+void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, const uint32_t* const* d_cols,
+                                qm31 rc, uint32_t n_batches, const StwoCirclePointQM31* points,
+                                const uint32_t* sizes, const uint32_t* col_idx, const StwoQM31* values,
+                                uint32_t* const out[4]) {
+  require(is_pow2(domain_size) && domain_size >= 2, "accumulate_quotients: domain size must be a power of two >= 2");
+  uint32_t log_size = log2_exact(domain_size);
+  std::vector<quot::Batch> bs(n_batches);
+  std::vector<quot::Sample> ss;
+  uint32_t off = 0;
+  for (uint32_t b = 0; b < n_batches; b++) {
+    qm31 px = from_w(qm31w{points[b].x.a, points[b].x.b, points[b].x.c, points[b].x.d});
+    qm31 py = from_w(qm31w{points[b].y.a, points[b].y.b, points[b].y.c, points[b].y.d});
+    qm31 c = qsub(qm31{py.x, cneg(py.y)}, py);
+    quot::Batch& B = bs[b];
+    B.first = (uint32_t)ss.size();
+    B.count = sizes[b];
+    B.c = c;
+    B.a_sum = qzero();
+    B.b_sum = qzero();
+    B.coeff = qpow(rc, sizes[b]);
+    B.prx = px.x;
+    B.pix = px.y;
+    B.pry = py.x;
+    B.piy = py.y;
+    qm31 alpha = qone();
+    for (uint32_t j = 0; j < sizes[b]; j++) {
+      const StwoQM31& w = values[off + j];
+      qm31 v = {{w.a, w.b}, {w.c, w.d}};
+      alpha = qmul(alpha, rc);
+      qm31 a = qsub(qm31{v.x, cneg(v.y)}, v);
+      qm31 bb = qsub(qmul(v, c), qmul(a, py));
+      B.a_sum = qadd(B.a_sum, qmul(alpha, a));
+      B.b_sum = qadd(B.b_sum, qmul(alpha, bb));
+      ss.push_back(quot::Sample{col_idx[off + j], 0, alpha});
+    }
+    off += sizes[b];
+  }
+  if (ss.empty()) ss.push_back(quot::Sample{0, 0, qzero()});
+  if (bs.empty()) bs.push_back(quot::Batch{});
+  DevBuf dss = upload(ss.data(), ss.size() * sizeof(quot::Sample));
+  DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
+  timed("accumulate_quotients", [&] {
+    quot::accumulate_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().stream>>>(
+        hi, hs, log_size, d_cols, dss.as<quot::Sample>(), dbs.as<quot::Batch>(), n_batches, out[0], out[1], out[2],
+        out[3]);
+  });
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" void accumulate_quotients(uint32_t half_coset_initial_index, uint32_t half_coset_step_size,
+                                     uint32_t domain_size, uint32_t* const* columns, size_t number_of_columns,
+                                     StwoQM31 random_coeff, const uint32_t* sample_points,
+                                     const uint32_t* sample_column_indexes, uint32_t sample_column_indexes_size,
+                                     const StwoQM31* sample_column_values,
+                                     const uint32_t* sample_column_and_values_sizes, uint32_t sample_size,
+                                     uint32_t* result_column_0, uint32_t* result_column_1, uint32_t* result_column_2,
+                                     uint32_t* result_column_3, uint32_t flattened_line_coeffs_size) {
+  (void)flattened_line_coeffs_size;
+  guarded([&] {
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < sample_size; b++) total += sample_column_and_values_sizes[b];
+    require(total == sample_column_indexes_size, "accumulate_quotients: batch sizes do not sum to the index count");
+    for (uint64_t i = 0; i < total; i++)
+      require(sample_column_indexes[i] < number_of_columns, "accumulate_quotients: column index out of range");
+    uint32_t* out[4] = {result_column_0, result_column_1, result_column_2, result_column_3};
+    accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
+                               (const uint32_t* const*)columns, qm31{{random_coeff.a, random_coeff.b},
+                                                                     {random_coeff.c, random_coeff.d}},
+                               sample_size, (const StwoCirclePointQM31*)sample_points, sample_column_and_values_sizes,
+                               sample_column_indexes, sample_column_values, out);
+    sync();
+  });
+}

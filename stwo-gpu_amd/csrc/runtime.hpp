@@ -1,0 +1,134 @@
+// Internal runtime of libstwo_hip: per-device context (stream, caching pool,
+// twiddle cache), error recording, pointer-table uploads and launch timing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/stwo_hip.h"
+
+namespace sh {
+
+// ---- error recording (thread-local, printed once per failure) ----
+void set_error(int status, const char* fmt, ...);
+int last_status();
+struct Failure {  // thrown inside the library, converted to a status at the ABI edge
+  int status;
+};
+[[noreturn]] void fail(int status, const char* fmt, ...);
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(STWO_HIP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+inline void require(bool ok, const char* fmt, const char* detail = "") {
+  if (!ok) fail(STWO_HIP_ERR_ARG, fmt, detail);
+}
+
+// ---- caching device allocator --------------------------------------------
+// All library work runs on one stream, so a block returned to the cache can be
+// handed out again at once: later work on the stream is ordered after the
+// block's previous users.
+class Pool {
+ public:
+  void* alloc(size_t bytes);
+  void free(void* p);
+  void release_cached();  // synchronises the stream, then returns cached blocks to HIP
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  static size_t round(size_t b);
+  std::unordered_map<void*, size_t> live_;
+  std::multimap<size_t, void*> cache_;
+  size_t cached_ = 0;
+};
+
+struct TwiddleKey {
+  uint32_t initial, step, log;
+  bool operator<(const TwiddleKey& o) const {
+    return initial != o.initial ? initial < o.initial : step != o.step ? step < o.step : log < o.log;
+  }
+};
+struct TwiddleEntry {
+  uint32_t* tw;
+  uint32_t* itw;
+};
+
+struct TimedLaunch {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  std::map<TwiddleKey, TwiddleEntry> twiddles;
+  bool kernel_timing = false;
+  std::vector<TimedLaunch> pending;
+  std::map<std::string, std::pair<uint64_t, double>> stats;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+  hipDeviceProp_t props;
+};
+
+// Lock the library, make sure the context exists; every ABI function holds this.
+std::unique_lock<std::recursive_mutex> lock();
+Context& ctx();
+void sync();  // synchronise the library stream (resolves pending kernel timings)
+
+// Device buffer from the pool, returned to the pool on scope exit.
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { p = ctx().pool.alloc(bytes); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  ~DevBuf() {
+    if (p) ctx().pool.free(p);
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+// Copy a host array to a fresh pooled device buffer (stream-ordered).
+DevBuf upload(const void* host, size_t bytes);
+
+// Launch bookkeeping: when kernel timing is on, bracket the launch with events.
+void launch_begin(const char* name);
+void launch_end(const char* name);
+template <class F>
+inline void timed(const char* name, F&& f) {
+  launch_begin(name);
+  f();
+  hip_check(hipGetLastError(), name);
+  launch_end(name);
+}
+
+// Run `body` converting failures into a status; used at every ABI entry.
+template <class F>
+inline int guarded(F&& body) {
+  try {
+    auto g = lock();
+    body();
+    return STWO_HIP_OK;
+  } catch (const Failure& f) {
+    return f.status;
+  }
+}
+
+inline uint32_t log2_exact(uint64_t n) {
+  uint32_t l = 0;
+  while ((1ull << l) < n) l++;
+  return l;
+}
+inline bool is_pow2(uint64_t n) { return n && !(n & (n - 1)); }
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace sh

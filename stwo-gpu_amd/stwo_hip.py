@@ -1,0 +1,558 @@
+"""Python binding of libstwo_hip.so (the gfx950 Stwo backend) over its C ABI.
+
+The device-buffer classes and functions mirror the reference crate's Rust
+surface (stwo_gpu_backend/src/cuda/*_vec.rs and the trait impls in
+stwo_gpu_backend/src/{column,field,poly,blake2s,fri,quotient,accumulation}.rs)
+so tests read like the reference's own.  Every call goes to the HIP library;
+there is no CPU fallback: loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+P = (1 << 31) - 1
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libstwo_hip.so")
+
+u32 = C.c_uint32
+u64 = C.c_uint64
+sz = C.c_size_t
+u32p = C.POINTER(C.c_uint32)
+vp = C.c_void_p
+
+
+class QM31(C.Structure):
+    _fields_ = [("a", u32), ("b", u32), ("c", u32), ("d", u32)]
+
+
+class PointM31(C.Structure):
+    _fields_ = [("x", u32), ("y", u32)]
+
+
+class PointQM31(C.Structure):
+    _fields_ = [("x", QM31), ("y", QM31)]
+
+
+class StwoHipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise StwoHipError(f"{LIB_PATH} is not built (run python stwo-gpu_amd/build.py)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    pp = C.POINTER(vp)
+    sig = {
+        # reference FFI (bindings.rs)
+        "copy_uint32_t_vec_from_device_to_host": (None, [vp, vp, u32]),
+        "copy_uint32_t_vec_from_host_to_device": (vp, [vp, u32]),
+        "copy_uint32_t_vec_from_device_to_device": (vp, [vp, vp, u32]),
+        "cuda_malloc_uint32_t": (vp, [u32]),
+        "cuda_malloc_blake_2s_hash": (vp, [sz]),
+        "cuda_alloc_zeroes_uint32_t": (vp, [u32]),
+        "cuda_alloc_zeroes_blake_2s_hash": (vp, [sz]),
+        "cuda_free_memory": (None, [vp]),
+        "bit_reverse_base_field": (None, [vp, sz]),
+        "bit_reverse_secure_field": (None, [vp, sz]),
+        "batch_inverse_base_field": (None, [vp, vp, sz]),
+        "batch_inverse_secure_field": (None, [vp, vp, sz]),
+        "sort_values_and_permute_with_bit_reverse_order": (vp, [vp, sz]),
+        "precompute_twiddles": (vp, [PointM31, PointM31, sz]),
+        "interpolate": (None, [u32, vp, vp, u32, u32]),
+        "interpolate_columns": (None, [u32, pp, vp, u32, u32, u32]),
+        "evaluate": (None, [u32, vp, vp, u32, u32]),
+        "evaluate_columns": (None, [u32p, pp, vp, u32, u32, u32p]),
+        "eval_at_point": (QM31, [vp, u32, QM31, QM31]),
+        "fold_line": (None, [vp, sz, sz, pp, QM31, pp]),
+        "fold_circle_into_line": (None, [vp, sz, sz, pp, QM31, pp]),
+        "accumulate": (None, [u32, pp, pp]),
+        "commit_on_first_layer": (None, [sz, sz, vp, vp]),
+        "commit_on_layer_with_previous": (None, [sz, sz, vp, vp, vp]),
+        "copy_blake_2s_hash_vec_from_host_to_device": (vp, [vp, sz]),
+        "copy_blake_2s_hash_vec_from_device_to_host": (None, [vp, vp, sz]),
+        "copy_blake_2s_hash_vec_from_device_to_device": (None, [vp, vp, sz]),
+        "copy_device_pointer_vec_from_host_to_device": (vp, [pp, sz]),
+        "accumulate_quotients": (None, [u32, u32, u32, vp, sz, QM31, vp, u32p, u32, C.POINTER(QM31), u32p,
+                                        u32, vp, vp, vp, vp, u32]),
+        "evaluate_wide_fibonacci_constraint_quotients_on_domain": (None, [vp, vp, vp, vp, pp, vp, vp, u32, u32]),
+        # extensions
+        "stwo_hip_init": (C.c_int, [C.c_int]),
+        "stwo_hip_last_status": (C.c_int, []),
+        "stwo_hip_last_error": (C.c_char_p, []),
+        "stwo_hip_clear_error": (None, []),
+        "stwo_hip_synchronize": (C.c_int, []),
+        "stwo_hip_stream": (vp, []),
+        "stwo_hip_malloc": (C.c_int, [C.POINTER(vp), sz]),
+        "stwo_hip_free": (C.c_int, [vp]),
+        "stwo_hip_memcpy": (C.c_int, [vp, vp, sz]),
+        "stwo_hip_release_cached_memory": (C.c_int, []),
+        "stwo_hip_fill_uniform_m31": (C.c_int, [vp, u64, u64, u64]),
+        "stwo_hip_twiddles": (C.c_int, [u32, u32, u32, C.POINTER(vp), C.POINTER(vp)]),
+        "stwo_hip_release_twiddles": (C.c_int, []),
+        "stwo_hip_interpolate_columns_async": (C.c_int, [pp, u32p, u32, vp, u64]),
+        "stwo_hip_evaluate_columns_async": (C.c_int, [pp, u32p, pp, u32p, u32, vp, u64]),
+        "stwo_hip_interpolate_columns_oop_async": (C.c_int, [pp, pp, u32p, u32, vp, u64]),
+        "stwo_hip_merkle_commit_async": (C.c_int, [u32, pp, u32p, pp, u32]),
+        "stwo_hip_merkle_root": (C.c_int, [vp, vp]),
+        "stwo_hip_timer_begin": (C.c_int, []),
+        "stwo_hip_timer_end": (C.c_int, [C.POINTER(C.c_float)]),
+        "stwo_hip_kernel_timing": (C.c_int, [C.c_int]),
+        "stwo_hip_kernel_stats": (C.c_int, [C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]),
+        "stwo_hip_kernel_stats_reset": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def exported_symbols():
+    """Names every header declaration must resolve to (checked by the CPU tests)."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(HERE), "include", "stwo_hip.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", hdr)
+    skip = {"if", "for", "while", "sizeof", "return"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def check(status=None):
+    """Raise if the last library call failed."""
+    L = lib()
+    st = L.stwo_hip_last_status() if status is None else status
+    if st != 0:
+        msg = L.stwo_hip_last_error().decode()
+        L.stwo_hip_clear_error()
+        raise StwoHipError(f"stwo_hip error {st}: {msg}")
+
+
+def _c(status):
+    check(status)
+
+
+def _ptr_array(ptrs):
+    return (vp * len(ptrs))(*ptrs)
+
+
+def qm31(v) -> QM31:
+    return QM31(*[int(x) for x in v])
+
+
+def from_qm31(q: QM31):
+    return (q.a, q.b, q.c, q.d)
+
+
+# ----------------------------------------------------------------- device buffers
+class DeviceBuffer:
+    """Owning device allocation (freed with cuda_free_memory, like the Rust Drop impls)."""
+
+    def __init__(self, ptr, nbytes):
+        if not ptr and nbytes:
+            check()
+            raise StwoHipError("allocation failed")
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def free(self):
+        if self.ptr:
+            lib().cuda_free_memory(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class BaseFieldVec(DeviceBuffer):
+    """stwo_gpu_backend/src/cuda/base_field_vec.rs:7-79: device M31 column."""
+
+    def __init__(self, ptr, size):
+        super().__init__(ptr, 4 * size)
+        self.size = size
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        ptr = lib().copy_uint32_t_vec_from_host_to_device(a.ctypes.data, a.size)
+        check()
+        return cls(ptr, a.size)
+
+    @classmethod
+    def uninitialized(cls, size):
+        ptr = lib().cuda_malloc_uint32_t(size)
+        check()
+        return cls(ptr, size)
+
+    @classmethod
+    def zeros(cls, size):
+        ptr = lib().cuda_alloc_zeroes_uint32_t(size)
+        check()
+        return cls(ptr, size)
+
+    def to_numpy(self):
+        out = np.empty(self.size, np.uint32)
+        lib().copy_uint32_t_vec_from_device_to_host(self.ptr, out.ctypes.data, self.size)
+        check()
+        return out
+
+    def __len__(self):
+        return self.size
+
+
+class SecureFieldVec(DeviceBuffer):
+    """secure_field_vec.rs:7-72: device QM31 vector, AoS (4 words per element)."""
+
+    def __init__(self, ptr, size):
+        super().__init__(ptr, 16 * size)
+        self.size = size
+
+    @classmethod
+    def from_numpy(cls, a):  # shape (n, 4)
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1)
+        ptr = lib().copy_uint32_t_vec_from_host_to_device(a.ctypes.data, a.size)
+        check()
+        return cls(ptr, a.size // 4)
+
+    @classmethod
+    def uninitialized(cls, size):
+        ptr = lib().cuda_malloc_uint32_t(4 * size)
+        check()
+        return cls(ptr, size)
+
+    def to_numpy(self):
+        out = np.empty(4 * self.size, np.uint32)
+        lib().copy_uint32_t_vec_from_device_to_host(self.ptr, out.ctypes.data, 4 * self.size)
+        check()
+        return out.reshape(-1, 4)
+
+
+class Blake2sHashVec(DeviceBuffer):
+    """blake_2s_hash_vec.rs:5-72: device vector of 32-byte hashes."""
+
+    def __init__(self, ptr, size):
+        super().__init__(ptr, 32 * size)
+        self.size = size
+
+    @classmethod
+    def uninitialized(cls, size):
+        ptr = lib().cuda_malloc_blake_2s_hash(size)
+        check()
+        return cls(ptr, size)
+
+    def to_numpy(self):
+        out = np.empty(8 * self.size, np.uint32)
+        lib().copy_blake_2s_hash_vec_from_device_to_host(self.ptr, out.ctypes.data, self.size)
+        check()
+        return out.reshape(-1, 8)
+
+
+class SecureColumn:
+    """SecureColumnByCoords<CudaBackend>: four M31 device columns (secure_column.rs:6-57)."""
+
+    def __init__(self, cols):
+        self.columns = list(cols)
+
+    @classmethod
+    def from_numpy(cls, a):  # shape (4, n)
+        return cls([BaseFieldVec.from_numpy(a[k]) for k in range(4)])
+
+    @classmethod
+    def uninitialized(cls, n):
+        return cls([BaseFieldVec.uninitialized(n) for _ in range(4)])
+
+    @classmethod
+    def zeros(cls, n):
+        return cls([BaseFieldVec.zeros(n) for _ in range(4)])
+
+    def ptrs(self):
+        return _ptr_array([c.ptr for c in self.columns])
+
+    def to_numpy(self):
+        return np.stack([c.to_numpy() for c in self.columns])
+
+    def __len__(self):
+        return self.columns[0].size
+
+
+# ----------------------------------------------------------------- cosets / domains
+@dataclass(frozen=True)
+class Coset:
+    initial_index: int
+    step_size: int
+    log_size: int
+
+    def size(self):
+        return 1 << self.log_size
+
+    def double(self):
+        m = (1 << 31) - 1
+        return Coset((2 * self.initial_index) & m, (2 * self.step_size) & m, self.log_size - 1)
+
+
+def canonic_half_coset(log_size) -> Coset:
+    """CanonicCoset(log_size).half_coset (Coset::half_odds(log_size - 1))."""
+    return Coset(1 << (30 - log_size), (1 << (32 - log_size)) & ((1 << 31) - 1), log_size - 1)
+
+
+def canonic_coset(log_size) -> Coset:
+    """CanonicCoset(log_size).coset (Coset::odds(log_size))."""
+    return Coset(1 << (30 - log_size), (1 << (31 - log_size)) & ((1 << 31) - 1), log_size)
+
+
+def _pt_mul(p, q):
+    return ((p[0] * q[0] - p[1] * q[1]) % P, (p[0] * q[1] + p[1] * q[0]) % P)
+
+
+def point_at_index(i):
+    r, g = (1, 0), (2, 1268011823)
+    i &= (1 << 31) - 1
+    while i:
+        if i & 1:
+            r = _pt_mul(r, g)
+        g = _pt_mul(g, g)
+        i >>= 1
+    return r
+
+
+@dataclass
+class TwiddleTree:
+    """TwiddleTree<CudaBackend> (poly.rs:186-190)."""
+    root_coset: Coset
+    twiddles: BaseFieldVec
+    itwiddles: BaseFieldVec
+
+
+# ----------------------------------------------------------------- backend ops
+def bit_reverse_column(col):
+    if isinstance(col, SecureFieldVec):
+        lib().bit_reverse_secure_field(col.ptr, col.size)
+    else:
+        lib().bit_reverse_base_field(col.ptr, col.size)
+    check()
+
+
+def batch_inverse(src, dst):
+    if isinstance(src, SecureFieldVec):
+        lib().batch_inverse_secure_field(src.ptr, dst.ptr, src.size)
+    else:
+        lib().batch_inverse_base_field(src.ptr, dst.ptr, src.size)
+    check()
+
+
+def new_canonical_ordered(values: BaseFieldVec) -> BaseFieldVec:
+    ptr = lib().sort_values_and_permute_with_bit_reverse_order(values.ptr, values.size)
+    check()
+    return BaseFieldVec(ptr, values.size)
+
+
+def precompute_twiddles(coset: Coset) -> TwiddleTree:
+    """poly.rs:170-192: tree from the coset's initial/step points, then batch inverse."""
+    ini = point_at_index(coset.initial_index)
+    step = point_at_index(coset.step_size)
+    ptr = lib().precompute_twiddles(PointM31(*ini), PointM31(*step), coset.size())
+    check()
+    tw = BaseFieldVec(ptr, coset.size())
+    itw = BaseFieldVec.uninitialized(coset.size())
+    batch_inverse(tw, itw)
+    return TwiddleTree(coset, tw, itw)
+
+
+def interpolate(values: BaseFieldVec, tree: TwiddleTree):
+    lib().interpolate(values.size // 2, values.ptr, tree.itwiddles.ptr, tree.itwiddles.size, values.size)
+    check()
+
+
+def interpolate_columns(cols, tree: TwiddleTree):
+    n = cols[0].size
+    lib().interpolate_columns(n // 2, _ptr_array([c.ptr for c in cols]), tree.itwiddles.ptr, tree.itwiddles.size,
+                              len(cols), n)
+    check()
+
+
+def evaluate(values: BaseFieldVec, tree: TwiddleTree):
+    lib().evaluate(values.size // 2, values.ptr, tree.twiddles.ptr, tree.twiddles.size, values.size)
+    check()
+
+
+def evaluate_columns(cols, tree: TwiddleTree):
+    sizes = np.array([c.size for c in cols], np.uint32)
+    halves = sizes // 2
+    lib().evaluate_columns(halves.ctypes.data_as(u32p), _ptr_array([c.ptr for c in cols]), tree.twiddles.ptr,
+                           tree.twiddles.size, len(cols), sizes.ctypes.data_as(u32p))
+    check()
+
+
+def eval_at_point(coeffs: BaseFieldVec, px, py):
+    r = lib().eval_at_point(coeffs.ptr, coeffs.size, qm31(px), qm31(py))
+    check()
+    return from_qm31(r)
+
+
+def commit_on_layer(log_size, prev, columns):
+    """MerkleOps::commit_on_layer (blake2s.rs:19-97)."""
+    size = 1 << log_size
+    out = Blake2sHashVec.uninitialized(size)
+    table = lib().copy_device_pointer_vec_from_host_to_device(_ptr_array([c.ptr for c in columns] or [None]),
+                                                               max(len(columns), 1))
+    check()
+    try:
+        if prev is None:
+            lib().commit_on_first_layer(size, len(columns), table, out.ptr)
+        else:
+            lib().commit_on_layer_with_previous(size, len(columns), table, prev.ptr, out.ptr)
+        check()
+    finally:
+        lib().cuda_free_memory(table)
+    return out
+
+
+def fold_line(src: SecureColumn, alpha, tree: TwiddleTree) -> SecureColumn:
+    """FriOps::fold_line (fri.rs:16-43): twiddle_offset = itwiddles.len - n."""
+    n = len(src)
+    dst = SecureColumn.uninitialized(n // 2)
+    lib().fold_line(tree.itwiddles.ptr, tree.itwiddles.size - n, n, src.ptrs(), qm31(alpha), dst.ptrs())
+    check()
+    return dst
+
+
+def fold_circle_into_line(dst: SecureColumn, src: SecureColumn, alpha, tree: TwiddleTree):
+    """FriOps::fold_circle_into_line (fri.rs:45-66): twiddle_offset = root.size - dst.size."""
+    n = len(src)
+    lib().fold_circle_into_line(tree.itwiddles.ptr, tree.root_coset.size() - len(dst), n, src.ptrs(), qm31(alpha),
+                                dst.ptrs())
+    check()
+
+
+def accumulate(left: SecureColumn, right: SecureColumn):
+    lib().accumulate(len(left), left.ptrs(), right.ptrs())
+    check()
+
+
+def accumulate_quotients(domain_half_coset: Coset, columns, random_coeff, sample_batches) -> SecureColumn:
+    """QuotientOps::accumulate_quotients (quotient.rs:17-115).
+    sample_batches: list of ((px4, py4), [(column_index, value4), ...])."""
+    n = 2 * domain_half_coset.size()
+    out = SecureColumn.uninitialized(n)
+    table = lib().copy_device_pointer_vec_from_host_to_device(_ptr_array([c.ptr for c in columns]), len(columns))
+    check()
+    pts = (PointQM31 * max(1, len(sample_batches)))()
+    for i, ((px, py), _) in enumerate(sample_batches):
+        pts[i] = PointQM31(qm31(px), qm31(py))
+    idx = np.array([c for _, cv in sample_batches for c, _ in cv] or [0], np.uint32)
+    vals = (QM31 * max(1, len(idx)))(*[qm31(v) for _, cv in sample_batches for _, v in cv])
+    sizes = np.array([len(cv) for _, cv in sample_batches] or [0], np.uint32)
+    total = int(sum(len(cv) for _, cv in sample_batches))
+    try:
+        lib().accumulate_quotients(domain_half_coset.initial_index, domain_half_coset.step_size, n, table,
+                                   len(columns), qm31(random_coeff), C.cast(pts, vp), idx.ctypes.data_as(u32p),
+                                   total, vals, sizes.ctypes.data_as(u32p), len(sample_batches),
+                                   *[c.ptr for c in out.columns], 3 * total)
+        check()
+    finally:
+        lib().cuda_free_memory(table)
+    return out
+
+
+def wide_fib_constraint_quotients(out: SecureColumn, trace, powers: SecureFieldVec, denom_inv: BaseFieldVec):
+    lib().evaluate_wide_fibonacci_constraint_quotients_on_domain(
+        *[c.ptr for c in out.columns], _ptr_array([c.ptr for c in trace]), powers.ptr, denom_inv.ptr,
+        len(out), len(trace))
+    check()
+
+
+# ----------------------------------------------------------------- extensions
+def synchronize():
+    _c(lib().stwo_hip_synchronize())
+
+
+def cached_twiddles(coset: Coset):
+    tw, itw = vp(), vp()
+    _c(lib().stwo_hip_twiddles(coset.initial_index, coset.step_size, coset.log_size, C.byref(tw), C.byref(itw)))
+    return tw.value, itw.value
+
+
+def fill_uniform(col: BaseFieldVec, seed, offset=0):
+    _c(lib().stwo_hip_fill_uniform_m31(col.ptr, col.size, seed, offset))
+
+
+def interpolate_columns_async(cols, log_sizes, itw_ptr, tree_len):
+    logs = np.ascontiguousarray(log_sizes, np.uint32)
+    _c(lib().stwo_hip_interpolate_columns_async(_ptr_array([c.ptr for c in cols]), logs.ctypes.data_as(u32p),
+                                                len(cols), itw_ptr, tree_len))
+
+
+def interpolate_columns_oop_async(src, dst, log_sizes, itw_ptr, tree_len):
+    logs = np.ascontiguousarray(log_sizes, np.uint32)
+    _c(lib().stwo_hip_interpolate_columns_oop_async(_ptr_array([c.ptr for c in src]), _ptr_array([c.ptr for c in dst]),
+                                                    logs.ctypes.data_as(u32p), len(src), itw_ptr, tree_len))
+
+
+def evaluate_columns_async(coeffs, coeff_logs, dst, eval_logs, tw_ptr, tree_len):
+    cl = np.ascontiguousarray(coeff_logs, np.uint32)
+    el = np.ascontiguousarray(eval_logs, np.uint32)
+    _c(lib().stwo_hip_evaluate_columns_async(_ptr_array([c.ptr for c in coeffs]), cl.ctypes.data_as(u32p),
+                                             _ptr_array([c.ptr for c in dst]), el.ctypes.data_as(u32p), len(coeffs),
+                                             tw_ptr, tree_len))
+
+
+class MerkleTree:
+    """All layers of a Blake2s Merkle tree over columns of mixed sizes."""
+
+    def __init__(self, cols, log_sizes=None):
+        if log_sizes is None:
+            log_sizes = [c.size.bit_length() - 1 for c in cols]
+        self.max_log = max(log_sizes)
+        self.layers = [Blake2sHashVec.uninitialized(1 << k) for k in range(self.max_log + 1)]
+        self._cols = cols
+        self._logs = np.ascontiguousarray(log_sizes, np.uint32)
+
+    def commit_async(self):
+        _c(lib().stwo_hip_merkle_commit_async(len(self._cols), _ptr_array([c.ptr for c in self._cols]),
+                                              self._logs.ctypes.data_as(u32p),
+                                              _ptr_array([l.ptr for l in self.layers]), self.max_log))
+        return self
+
+    def root(self):
+        out = np.zeros(8, np.uint32)
+        _c(lib().stwo_hip_merkle_root(self.layers[0].ptr, out.ctypes.data))
+        return out
+
+
+def timer_begin():
+    _c(lib().stwo_hip_timer_begin())
+
+
+def timer_end():
+    ms = C.c_float()
+    _c(lib().stwo_hip_timer_end(C.byref(ms)))
+    return ms.value
+
+
+def kernel_timing(on):
+    _c(lib().stwo_hip_kernel_timing(1 if on else 0))
+
+
+def kernel_stats(name):
+    n, ms = u64(), C.c_double()
+    _c(lib().stwo_hip_kernel_stats(name.encode(), C.byref(n), C.byref(ms)))
+    return n.value, ms.value
+
+
+def kernel_stats_reset():
+    _c(lib().stwo_hip_kernel_stats_reset())

@@ -1,0 +1,402 @@
+"""GPU parity: every backend op through the C ABI against the oracle, bit-exact.
+
+Mirrors the reference's own tests (stwo_gpu_backend/src/*.rs `mod tests`), with
+CpuBackend replaced by the oracle; sizes are chosen so the oracle finishes in
+seconds.  Full-size configurations are covered by property tests in
+test_gpu_properties.py.
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+import stwo_hip as S
+
+pytestmark = pytest.mark.gpu
+P = S.P
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_fixtures.json")))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    st = S.lib().stwo_hip_init(0)
+    S.check(st)
+    yield
+
+
+def rand_m31(n, seed):
+    return np.random.default_rng(seed).integers(0, P, n, dtype=np.uint32)
+
+
+def rand_qm31(rng):
+    return tuple(rng.randrange(P) for _ in range(4))
+
+
+# ------------------------------------------------------------------ column.rs:156-186
+@pytest.mark.parametrize("log_n", [1, 5, 10, 11, 16, 20])
+def test_bit_reverse_base_field(log_n):
+    data = np.arange(1 << log_n, dtype=np.uint32)
+    col = S.BaseFieldVec.from_numpy(data)
+    S.bit_reverse_column(col)
+    assert (col.to_numpy() == O.bit_reverse(data)).all()
+
+
+@pytest.mark.parametrize("log_n", [3, 12, 16])
+def test_bit_reverse_secure_field(log_n):
+    data = np.arange(1, 1 + 4 * (1 << log_n), dtype=np.uint32).reshape(-1, 4)
+    v = S.SecureFieldVec.from_numpy(data)
+    S.bit_reverse_column(v)
+    assert (v.to_numpy().reshape(-1) == O.bit_reverse_qm31(data.reshape(-1))).all()
+
+
+# ------------------------------------------------------------------ field.rs:41-78
+@pytest.mark.parametrize("n", [1, 100, 1 << 12, (1 << 20) + 3])
+def test_batch_inverse_basefield(n):
+    src = np.arange(1, n + 1, dtype=np.uint32)
+    a, d = S.BaseFieldVec.from_numpy(src), S.BaseFieldVec.uninitialized(n)
+    S.batch_inverse(a, d)
+    assert (d.to_numpy() == O.batch_inverse_m31(src)).all()
+
+
+def test_batch_inverse_secure_field():
+    n = 1 << 16
+    src = np.arange(1, 4 * n + 1, dtype=np.uint32).reshape(-1, 4)
+    a, d = S.SecureFieldVec.from_numpy(src), S.SecureFieldVec.uninitialized(n)
+    S.batch_inverse(a, d)
+    assert (d.to_numpy().reshape(-1) == O.batch_inverse_qm31(src.reshape(-1))).all()
+
+
+def test_batch_inverse_zero_is_isolated():
+    src = np.array([3, 0, 5, 7, 0, 11], np.uint32)
+    a, d = S.BaseFieldVec.from_numpy(src), S.BaseFieldVec.uninitialized(6)
+    S.batch_inverse(a, d)
+    out = d.to_numpy()
+    assert out[1] == 0 and out[4] == 0
+    for i in (0, 2, 3, 5):
+        assert int(out[i]) * int(src[i]) % P == 1
+
+
+# ------------------------------------------------------------------ poly.rs:216-249
+def test_new_canonical_ordered():
+    n = 1 << 16
+    data = np.arange(n, dtype=np.uint32)
+    out = S.new_canonical_ordered(S.BaseFieldVec.from_numpy(data))
+    assert (out.to_numpy() == O.new_canonical_ordered(data)).all()
+
+
+@pytest.mark.parametrize("log_n", [5, 6, 13, 20])
+def test_precompute_twiddles(log_n):
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(coset)
+    tw, itw = O.precompute_twiddles(coset.initial_index, coset.step_size, coset.log_size)
+    assert (tree.twiddles.to_numpy() == tw).all()
+    assert (tree.itwiddles.to_numpy() == itw).all()
+
+
+def test_twiddle_fixture_and_cache():
+    f = FIX["interpolate_from_fib"]
+    rc = f["root_coset"]
+    tree = S.precompute_twiddles(S.Coset(rc["initial_index"], rc["step_size"], rc["log_size"]))
+    assert tree.twiddles.to_numpy().tolist() == f["twiddles"]
+    assert tree.itwiddles.to_numpy().tolist() == f["itwiddles"]
+    tw, itw = S.cached_twiddles(S.Coset(rc["initial_index"], rc["step_size"], rc["log_size"]))
+    host = np.zeros(64, np.uint32)
+    S.lib().copy_uint32_t_vec_from_device_to_host(tw, host.ctypes.data, 64)
+    assert host.tolist() == f["twiddles"]
+
+
+# ------------------------------------------------------------------ poly.rs:266-396
+@pytest.mark.parametrize("log_n", [1, 2, 3, 4, 8, 11, 12, 13, 14, 16, 17])
+def test_interpolate(log_n):
+    vals = np.arange(1, (1 << log_n) + 1, dtype=np.uint32)
+    ev = O.new_canonical_ordered(vals)
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(S.canonic_half_coset(log_n + 1))  # a larger tree serves the domain
+    col = S.BaseFieldVec.from_numpy(ev)
+    S.interpolate(col, tree)
+    _, itw = O.precompute_twiddles(*S.canonic_half_coset(log_n + 1).__dict__.values())
+    assert (col.to_numpy() == O.interpolate(ev, coset.initial_index, itw)).all()
+
+
+def test_interpolate_fixtures():
+    f = FIX["interpolate_from_fib"]
+    rc = f["root_coset"]
+    tree = S.TwiddleTree(S.Coset(rc["initial_index"], rc["step_size"], rc["log_size"]),
+                         S.BaseFieldVec.from_numpy(np.array(f["twiddles"], np.uint32)),
+                         S.BaseFieldVec.from_numpy(np.array(f["itwiddles"], np.uint32)))
+    vals = np.array(f["values"], np.uint32)
+    col = S.BaseFieldVec.from_numpy(vals)
+    S.interpolate(col, tree)
+    want = O.interpolate(vals, f["domain_half_coset"]["initial_index"], np.array(f["itwiddles"], np.uint32))
+    assert (col.to_numpy() == want).all()
+    # poly.rs:291-327
+    v2 = O.new_canonical_ordered(np.array(FIX["interpolate_2_values"], np.uint32))
+    col2 = S.BaseFieldVec.from_numpy(v2)
+    S.interpolate(col2, tree)
+    assert (col2.to_numpy() == O.interpolate(v2, 1 << 25, np.array(f["itwiddles"], np.uint32))).all()
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 7, 12, 15, 18])
+def test_evaluate(log_n):
+    coeffs = rand_m31(1 << log_n, log_n)
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(coset)
+    col = S.BaseFieldVec.from_numpy(coeffs)
+    S.evaluate(col, tree)
+    tw, _ = O.precompute_twiddles(coset.initial_index, coset.step_size, coset.log_size)
+    assert (col.to_numpy() == O.evaluate(coeffs, coset.initial_index, tw)).all()
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 11, 12, 16, 20])
+def test_eval_at_point(log_n):
+    rng = random.Random(log_n)
+    coeffs = rand_m31(1 << log_n, 100 + log_n)
+    px, py = rand_qm31(rng), rand_qm31(rng)  # any QM31 pair: the fold does not need the circle equation
+    got = S.eval_at_point(S.BaseFieldVec.from_numpy(coeffs), px, py)
+    assert got == O.eval_at_point(coeffs, px, py)
+
+
+# ------------------------------------------------------------------ poly.rs:616-706
+def test_interpolate_columns():
+    log_n, n_cols = 9, 256
+    vals = np.arange(1, (1 << log_n) + 1, dtype=np.uint32)
+    ev = O.new_canonical_ordered(vals)
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(coset)
+    cols = [S.BaseFieldVec.from_numpy(ev) for _ in range(n_cols)]
+    S.interpolate_columns(cols, tree)
+    want = O.interpolate(ev, coset.initial_index, tree.itwiddles.to_numpy())
+    for c in cols[:: 37]:
+        assert (c.to_numpy() == want).all()
+
+
+def test_evaluate_columns_blowup_4():
+    log_n, n_cols, blow = 9, 256, 2
+    vals = np.arange(1, (1 << log_n) + 1, dtype=np.uint32)
+    big = S.canonic_half_coset(log_n + blow)
+    tw, itw = O.precompute_twiddles(big.initial_index, big.step_size, big.log_size)
+    coeffs = O.interpolate(O.new_canonical_ordered(vals), S.canonic_half_coset(log_n).initial_index, itw)
+    tree = S.precompute_twiddles(big)
+    ext = np.zeros(1 << (log_n + blow), np.uint32)
+    ext[: 1 << log_n] = coeffs
+    cols = [S.BaseFieldVec.from_numpy(ext) for _ in range(n_cols)]
+    S.evaluate_columns(cols, tree)
+    want = O.evaluate(ext, big.initial_index, tw)
+    for c in cols[:: 51]:
+        assert (c.to_numpy() == want).all()
+
+
+@pytest.mark.parametrize("log_n,blow", [(3, 1), (11, 1), (12, 1), (13, 1), (16, 1), (16, 2), (17, 3)])
+def test_lde_async_matches_oracle(log_n, blow):
+    """Interpolate + evaluate-from-coefficients (no extend copy) on several columns."""
+    n_cols = 5
+    m = log_n + blow
+    root = S.canonic_half_coset(m)
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << log_n, 1000 * log_n + c) for c in range(n_cols)]
+    cols = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    ldes = [S.BaseFieldVec.uninitialized(1 << m) for _ in range(n_cols)]
+    S.interpolate_columns_async(cols, [log_n] * n_cols, itw, 1 << root.log_size)
+    S.evaluate_columns_async(cols, [log_n] * n_cols, ldes, [m] * n_cols, tw, 1 << root.log_size)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    ocoef, olde = O.lde_columns(evals, blow, otw, oitw)
+    for c in range(n_cols):
+        assert (cols[c].to_numpy() == ocoef[c]).all()
+        assert (ldes[c].to_numpy() == olde[c]).all()
+
+
+def test_lde_mixed_sizes_one_call():
+    logs = [4, 13, 9, 13, 16, 4, 11]
+    root = S.canonic_half_coset(17)
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << l, 77 + i) for i, l in enumerate(logs)]
+    cols = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    ldes = [S.BaseFieldVec.uninitialized(1 << (l + 1)) for l in logs]
+    S.interpolate_columns_async(cols, logs, itw, 1 << root.log_size)
+    S.evaluate_columns_async(cols, logs, ldes, [l + 1 for l in logs], tw, 1 << root.log_size)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    for i, l in enumerate(logs):
+        oc, ol = O.lde_columns([evals[i]], 1, otw, oitw)
+        assert (cols[i].to_numpy() == oc[0]).all()
+        assert (ldes[i].to_numpy() == ol[0]).all()
+
+
+# ------------------------------------------------------------------ blake2s.rs:109-192
+def _cols_row_times_col(n_cols, n):
+    return [np.array([(r * c) % P for r in range(n)], np.uint32) for c in range(n_cols)]
+
+
+def test_commit_on_first_layer_many_columns():
+    log_size = 11
+    cols = _cols_row_times_col(35, 1 << log_size)
+    got = S.commit_on_layer(log_size, None, [S.BaseFieldVec.from_numpy(c) for c in cols])
+    assert (got.to_numpy().reshape(-1) == O.merkle_commit_on_layer(log_size, None, cols)).all()
+
+
+def test_commit_on_layer_with_previous():
+    c12 = _cols_row_times_col(35, 1 << 12)
+    c11 = _cols_row_times_col(16, 1 << 11)
+    prev = S.commit_on_layer(12, None, [S.BaseFieldVec.from_numpy(c) for c in c12])
+    got = S.commit_on_layer(11, prev, [S.BaseFieldVec.from_numpy(c) for c in c11])
+    oprev = O.merkle_commit_on_layer(12, None, c12)
+    assert (got.to_numpy().reshape(-1) == O.merkle_commit_on_layer(11, oprev, c11)).all()
+    # a node layer with no columns
+    got2 = S.commit_on_layer(10, got, [])
+    assert (got2.to_numpy().reshape(-1) == O.merkle_commit_on_layer(10, O.merkle_commit_on_layer(11, oprev, c11), [])).all()
+
+
+@pytest.mark.parametrize("sizes", [[16] * 16, [12, 5, 12, 3, 9, 9, 0, 12], [17, 17, 3], [13] * 33 + [2, 1]])
+def test_merkle_tree_mixed_degree(sizes):
+    cols = [rand_m31(1 << s, 500 + i) for i, s in enumerate(sizes)]
+    tree = S.MerkleTree([S.BaseFieldVec.from_numpy(c) for c in cols], sizes).commit_async()
+    want = O.merkle_commit(cols)
+    for k in range(tree.max_log + 1):
+        assert (tree.layers[k].to_numpy().reshape(-1) == want[k]).all(), f"layer {k}"
+    assert (tree.root() == want[0]).all()
+
+
+# ------------------------------------------------------------------ fri.rs:99-310
+def _rand_secure(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, P, (4, n), dtype=np.uint32)
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 5, 13, 20])
+def test_fold_line(log_n):
+    src = _rand_secure(1 << log_n, log_n)
+    alpha = (1, 3, 5, 7)
+    dom = S.canonic_half_coset(log_n + 1)  # LineDomain(CanonicCoset(log_n + 1).half_coset())
+    tree = S.precompute_twiddles(dom)
+    got = S.fold_line(S.SecureColumn.from_numpy(src), alpha, tree)
+    assert (got.to_numpy() == O.fold_line(src, dom.initial_index, dom.step_size, alpha)).all()
+
+
+def test_fold_line_more_than_once():
+    log_n = 13
+    src = _rand_secure(1 << log_n, 99)
+    alpha = (1, 3, 5, 7)
+    dom = S.canonic_half_coset(log_n + 1)
+    tree = S.precompute_twiddles(dom)
+    g = S.SecureColumn.from_numpy(src)
+    o = src
+    d = dom
+    for _ in range(3):
+        g = S.fold_line(g, alpha, tree)
+        o = O.fold_line(o, d.initial_index, d.step_size, alpha)
+        d = d.double()
+    assert (g.to_numpy() == o).all()
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 4, 13, 20])
+def test_fold_circle_into_line(log_n):
+    src = _rand_secure(1 << log_n, 1000 + log_n)
+    dst0 = _rand_secure(1 << (log_n - 1), 2000 + log_n)
+    alpha = (1, 3, 5, 7)
+    hc = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(hc)  # line_domain.coset()
+    dst = S.SecureColumn.from_numpy(dst0)
+    S.fold_circle_into_line(dst, S.SecureColumn.from_numpy(src), alpha, tree)
+    assert (dst.to_numpy() == O.fold_circle_into_line(dst0, src, hc.initial_index, hc.step_size, alpha)).all()
+
+
+def test_fold_circle_into_line_fibonacci_fixture():
+    f = FIX["fold_circle_into_line_fibonacci"]
+    rc = f["root_coset"]
+    tree = S.TwiddleTree(S.Coset(rc["initial_index"], rc["step_size"], rc["log_size"]),
+                         S.BaseFieldVec.from_numpy(np.array(f["twiddles"], np.uint32)),
+                         S.BaseFieldVec.from_numpy(np.array(f["itwiddles"], np.uint32)))
+    dst0 = np.array(f["dst"], np.uint32)
+    src = np.array(f["src"], np.uint32)
+    dst = S.SecureColumn.from_numpy(dst0)
+    S.fold_circle_into_line(dst, S.SecureColumn.from_numpy(src), f["alpha"], tree)
+    hc = f["src_half_coset"]
+    assert (dst.to_numpy() == O.fold_circle_into_line(dst0, src, hc["initial_index"], hc["step_size"],
+                                                       f["alpha"])).all()
+
+
+# ------------------------------------------------------------------ quotient.rs:132-204
+def _point(rng):
+    import refmath as R
+    return R.random_qm31_point(rng)
+
+
+def test_accumulate_quotients_reference_shape():
+    # log 15 trace, blowup 1, 2 columns, 3 identical batches at one point (quotient.rs:132-204)
+    rng = random.Random(15)
+    log_n, blow = 15, 1
+    m = log_n + blow
+    root = S.canonic_half_coset(m)
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    e0 = np.arange(1 << log_n, dtype=np.uint32)
+    e1 = (2 * np.arange(1 << log_n, dtype=np.uint64) % P).astype(np.uint32)
+    coeffs, ldes = O.lde_columns([e0, e1], blow, otw, oitw)
+    pt = _point(rng)
+    a = O.eval_at_point(coeffs[0], *pt)
+    b = O.eval_at_point(coeffs[1], *pt)
+    batches = [(pt, [(0, a), (1, b)])] * 3
+    rc = (1, 2, 3, 4)
+    got = S.accumulate_quotients(root, [S.BaseFieldVec.from_numpy(c) for c in ldes], rc, batches)
+    want = O.accumulate_quotients(root.initial_index, root.step_size, m, ldes, rc, batches)
+    assert (got.to_numpy() == want).all()
+
+
+def test_accumulate_quotients_uneven_batches():
+    # batches of different sizes: the reference kernel's line-coefficient offset bug case
+    rng = random.Random(7)
+    m, n_cols = 12, 20
+    root = S.canonic_half_coset(m)
+    cols = [rand_m31(1 << m, 300 + i) for i in range(n_cols)]
+    batches = []
+    for b, size in enumerate([5, 1, 9, 2]):
+        idx = rng.sample(range(n_cols), size)
+        batches.append((_point(rng), [(c, rand_qm31(rng)) for c in idx]))
+    rc = rand_qm31(rng)
+    got = S.accumulate_quotients(root, [S.BaseFieldVec.from_numpy(c) for c in cols], rc, batches)
+    want = O.accumulate_quotients(root.initial_index, root.step_size, m, cols, rc, batches)
+    assert (got.to_numpy() == want).all()
+
+
+# ------------------------------------------------------------------ accumulation.rs:31-101
+def test_accumulation():
+    n = 2 << 12
+    left = S.SecureColumn.from_numpy(np.stack([np.full(n, v, np.uint32) for v in (1, 2, 3, 4)]))
+    right = S.SecureColumn.from_numpy(np.stack([np.full(n, v, np.uint32) for v in (5, 6, 7, 8)]))
+    S.accumulate(left, right)
+    assert (left.to_numpy()[:, 0] == [6, 8, 10, 12]).all() and (left.to_numpy()[:, -1] == [6, 8, 10, 12]).all()
+    ones = S.SecureColumn.from_numpy(np.ones((4, n), np.uint32))
+    pm1 = S.SecureColumn.from_numpy(np.full((4, n), P - 1, np.uint32))
+    S.accumulate(ones, pm1)
+    assert not ones.to_numpy().any()
+
+
+# ------------------------------------------------------------------ wide_fibonacci.rs:90-141
+def test_wide_fib_constraint_quotients():
+    rng = random.Random(3)
+    n_rows, n_cols = 1 << 13, 32
+    trace = [rand_m31(n_rows, 40 + i) for i in range(n_cols)]
+    powers = np.array([rand_qm31(rng) for _ in range(n_cols - 2)], np.uint32)
+    dinv = rand_m31(n_rows, 5)
+    out = S.SecureColumn.uninitialized(n_rows)
+    S.wide_fib_constraint_quotients(out, [S.BaseFieldVec.from_numpy(t) for t in trace],
+                                    S.SecureFieldVec.from_numpy(powers), S.BaseFieldVec.from_numpy(dinv))
+    assert (out.to_numpy() == O.wide_fib_constraint_quotients(trace, powers, dinv)).all()
+
+
+# ------------------------------------------------------------------ inputs / errors
+def test_uniform_generator_matches_oracle():
+    col = S.BaseFieldVec.uninitialized(1 << 18)
+    S.fill_uniform(col, 0x5EED0002, 12345)
+    S.synchronize()
+    assert (col.to_numpy() == O.fill_uniform_m31(1 << 18, 0x5EED0002, 12345)).all()
+
+
+def test_errors_are_reported_not_fatal():
+    col = S.BaseFieldVec.from_numpy(np.arange(12, dtype=np.uint32))
+    with pytest.raises(S.StwoHipError):
+        S.bit_reverse_column(col)  # not a power of two
+    S.bit_reverse_column(S.BaseFieldVec.from_numpy(np.arange(16, dtype=np.uint32)))  # still usable
