@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-sample-cols", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
@@ -95,6 +97,12 @@ def main():
     tree_len = 1 << root.log_size
     tree = S.MerkleTree(ldes, [LOG_N + LOG_BLOWUP] * C)
     S.synchronize()
+    if args.calibrate_copy:
+        # copy the first 256 evaluation columns' worth of bytes: 1 GiB read + 1 GiB written
+        a, b = S.BaseFieldVec.uninitialized(1 << 28), S.BaseFieldVec.uninitialized(1 << 28)
+        S.device_copy(b, a)
+        S.synchronize()
+        del a, b
 
     def step():
         S.interpolate_columns_oop_async(evals, coeffs, [LOG_N] * C, itw, tree_len)
@@ -139,6 +147,12 @@ def main():
     achieved = alg[dom] / (d_avg * 1e-3) / 1e9
     pipe = pipeline_bytes(C, n)
     root_hash = tree.root()
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "lde_merkle_256x2e20_blowup2_traffic.json")
+    if os.path.exists(tf):  # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)
+        rec = json.load(open(tf))["kernels"].get(dom)
+        if rec:
+            traffic = rec["read_bytes"] + rec["write_bytes"]
 
     out = {
         "metric": "LDE+Merkle M31 elems/s",
@@ -156,7 +170,7 @@ def main():
         "config": {"workload": "cfg2 LDE+commit: 256 M31 columns log 20 -> LDE log 21 (blowup 2) + Blake2s Merkle",
                    "columns": C, "log_size": LOG_N, "log_blowup": LOG_BLOWUP, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg[dom], "avg_launch_ms": d_avg},
         "pipeline": {"algorithmic_bytes_per_step": pipe,
                      "achieved_GBps": round(pipe / (ms_per_step * 1e-3) / 1e9, 1),

@@ -184,6 +184,8 @@ int stwo_hip_free(void *ptr);
 int stwo_hip_memcpy(void *dst, const void *src, size_t bytes); /* synchronous, any direction */
 int stwo_hip_release_cached_memory(void);
 
+/* Device-to-device copy by a plain 16-byte-per-lane kernel (async; 16-byte aligned sizes). */
+int stwo_hip_device_copy(void *dst, const void *src, size_t bytes);
 /* Synthetic input: dst[i] = uniform M31 value number (offset + i) of stream `seed` (async). */
 int stwo_hip_fill_uniform_m31(uint32_t *dst, uint64_t n, uint64_t seed, uint64_t offset);
 

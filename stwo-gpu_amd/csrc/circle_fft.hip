@@ -21,6 +21,7 @@
 //     zeros only duplicate values, so the first pass reads coeffs[idx mod 2^c]
 //     and skips those layers (no extend copy, no wasted butterflies).
 #include <algorithm>
+#include <type_traits>
 
 #include "field.hpp"
 #include "runtime.hpp"
@@ -44,224 +45,243 @@ struct PassArgs {
   uint32_t lob_count_log;  // log2 of the number of low blocks per column (a - t)
 };
 
-// Twiddle of layer `layer` (global) at butterfly-block index h.
-template <bool INV>
-__device__ __forceinline__ uint32_t twiddle(const uint32_t* __restrict__ tw, uint32_t log_n, uint32_t layer,
-                                            uint32_t h) {
-  if (layer == 0) {
-    uint32_t k2 = (h >> 2) << 1;
-    uint32_t sel = h & 3;  // [y, -y, -x, x] with (x, y) = (tw[2k], tw[2k+1])
-    uint32_t v = tw[k2 + ((sel == 0 || sel == 1) ? 1 : 0)];
-    return (sel == 1 || sel == 2) ? neg(v) : v;
-  }
-  uint32_t half = 1u << (log_n - 1);
-  uint32_t off = half - (half >> (layer - 1));
-  return tw[off + h];
+// Column pointers come from device tables; tell the compiler they are global memory so
+// it emits global_* rather than flat_* accesses.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SH_GLOBAL __attribute__((address_space(1)))
+#else
+#define SH_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ SH_GLOBAL T* as_global(T* p) {
+  return (SH_GLOBAL T*)p;
 }
 
-template <int B, int NT>
-struct Tile {
-  static constexpr int E = (1 << B) / NT;
-};
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
 
 // p (tile position) with W zero bits inserted at bit `at`
-__device__ __forceinline__ uint32_t spread(uint32_t gamma, uint32_t at, uint32_t W) {
-  uint32_t lowmask = (1u << at) - 1;
+template <int AT, int W>
+__device__ __forceinline__ uint32_t spread(uint32_t gamma) {
+  constexpr uint32_t lowmask = (1u << AT) - 1;
   return (gamma & lowmask) | ((gamma & ~lowmask) << W);
 }
 
-struct Geo {
-  uint32_t t, k, hb, a, log_n, multi;
-  uint32_t hi_tile, lob, col_group;
-  // global element index (within its column) and column of tile position p
-  __device__ __forceinline__ void locate(uint32_t p, uint32_t& col, uint32_t& idx) const {
-    uint32_t j = p & ((1u << t) - 1);
-    uint32_t m = (p >> t) & ((1u << k) - 1);
-    uint32_t r = p >> (t + k);
-    if (multi) {
-      col = (col_group << hb) + r;
-      idx = m;
+// Pass shape, fixed at compile time.  A tile holds 2^B positions p = [T low index
+// bits | K layer bits | HB high bits]:
+//   CONTIG  (a = 0):            T = 0, K = B, HB = 0 (one column, 2^B consecutive values)
+//   STRIDED (a > 0):            T = B - K, HB = 0 (2^T-word contiguous runs)
+//   MULTI   (a = 0, log_n = K): T = 0, HB = B - K bits selecting the column
+enum Kind { CONTIG = 0, STRIDED = 1, MULTI = 2 };
+
+template <bool INV, int B, int NT, int K, int KIND>
+struct Pass {
+  static constexpr int T = KIND == STRIDED ? B - K : 0;
+  static constexpr int HB = KIND == MULTI ? B - K : 0;
+  static constexpr int E = (1 << B) / NT;
+  static constexpr int NCH = (K + 2) / 3;
+
+  const PassArgs& A;
+  uint32_t* s;
+  uint32_t tile, col_group;
+  uint64_t block_base;  // index of tile position 0 (CONTIG / STRIDED)
+  uint32_t hi_tile;     // tile's high index bits (twiddle block index)
+  const SH_GLOBAL uint32_t* src;  // CONTIG / STRIDED: this block's column
+  SH_GLOBAL uint32_t* dst;
+
+  __device__ __forceinline__ Pass(const PassArgs& a, uint32_t* lds) : A(a), s(lds) {
+    col_group = blockIdx.x % A.n_groups;
+    tile = blockIdx.x / A.n_groups;
+    if constexpr (KIND == MULTI) {
+      block_base = 0;
+      hi_tile = 0;
+      src = nullptr;
+      dst = nullptr;
     } else {
-      col = col_group;
-      idx = ((((hi_tile << hb) + r) << (a + k)) | (m << a) | (lob << t) | j);
+      const uint32_t lob = tile & ((1u << A.lob_count_log) - 1);
+      hi_tile = tile >> A.lob_count_log;
+      block_base = ((uint64_t)hi_tile << (A.a + K)) | ((uint64_t)lob << T);
+      src = as_global(A.src[col_group]);
+      dst = as_global(A.dst[col_group]);
     }
   }
-  // butterfly-block index h for layer bit b (global layer a + b) of position p
-  __device__ __forceinline__ uint32_t h_of(uint32_t p, uint32_t b) const {
-    uint32_t m = (p >> t) & ((1u << k) - 1);
-    uint32_t r = p >> (t + k);
-    if (multi) return m >> (b + 1);
-    return (((hi_tile << hb) + r) << (k - b - 1)) | (m >> (b + 1));
-  }
-};
 
-// One radix-2^W stage on the thread's groups: layer bits [lo, lo + W) of the k
-// layer bits, active only below `limit` (k, or src_log - a for a forward first pass).
-template <bool INV, int B, int NT, int W>
-__device__ __forceinline__ void stage(uint32_t (&v)[(1 << B) / NT], const Geo& g, const uint32_t* __restrict__ tw,
-                                      uint32_t lo, uint32_t limit) {
-  constexpr int E = Tile<B, NT>::E;
-  constexpr int R = 1 << W;
-  constexpr int G = E / R;
-  const uint32_t tid = threadIdx.x;
+  // global index (within its column) of tile position p
+  __device__ __forceinline__ uint64_t index_of(uint32_t p) const {
+    if constexpr (KIND == MULTI) return p & ((1u << K) - 1);
+    else if constexpr (T == 0) return block_base + p;
+    else return block_base + ((uint64_t)(p >> T) << A.a) + (p & ((1u << T) - 1));
+  }
+  __device__ __forceinline__ uint32_t col_of(uint32_t p) const { return (col_group << HB) + (p >> K); }
+
+  // twiddle of layer bit b at the butterfly block containing position p
+  template <int BIT>
+  __device__ __forceinline__ uint32_t twiddle(const SH_GLOBAL uint32_t* lvl, uint32_t p) const {
+    const uint32_t m = (p >> T) & ((1u << K) - 1);
+    const uint32_t h = (hi_tile << (K - BIT - 1)) | (m >> (BIT + 1));
+    if constexpr (KIND != STRIDED && BIT == 0) {
+      // circle layer: [y, -y, -x, x] over pairs (x, y) of the first line level
+      const uint32_t k2 = (h >> 2) << 1, sel = h & 3;
+      const uint32_t v = lvl[k2 + (sel < 2 ? 1 : 0)];
+      return (sel == 1 || sel == 2) ? neg(v) : v;
+    } else {
+      return lvl[h];
+    }
+  }
+  // line level used by layer bit b (global layer a + b >= 1); level 0 for the circle layer
+  __device__ __forceinline__ const SH_GLOBAL uint32_t* level(uint32_t b) const {
+    const uint32_t layer = A.a + b;
+    if (layer == 0) return as_global(A.tw);
+    const uint32_t half = 1u << (A.log_n - 1);
+    return as_global(A.tw) + (half - (half >> (layer - 1)));
+  }
+
+  template <int LO, int W>
+  __device__ __forceinline__ void load(uint32_t (&v)[E]) const {
+    constexpr int R = 1 << W;
+    const uint32_t smask = (1u << A.src_log) - 1;
 #pragma unroll
-  for (int gi = 0; gi < G; gi++) {
-    uint32_t gamma = tid + NT * gi;
-    uint32_t pbase = spread(gamma, g.t + lo, W);
+    for (int gi = 0; gi < E / R; gi++) {
+      const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+      if constexpr (KIND == MULTI) {
+        const uint32_t col = col_of(pb);
+        const SH_GLOBAL uint32_t* c = col < A.n_cols ? as_global(A.src[col]) : nullptr;
 #pragma unroll
-    for (int s = 0; s < W; s++) {
-      const int bl = INV ? s : (W - 1 - s);
-      const uint32_t b = lo + bl;
-      if (b >= limit) continue;
-      const uint32_t layer = g.a + b;
+        for (int r = 0; r < R; r++) v[gi * R + r] = c ? c[(pb + (r << LO)) & smask] : 0u;
+      } else if constexpr (LO == 0 && T == 0 && W == 3) {
+        const uint64_t i0 = index_of(pb);
+        if (A.src_log >= 3) {
+          const SH_GLOBAL uint4* q = reinterpret_cast<const SH_GLOBAL uint4*>(src + (i0 & smask));
+          uint4 x0 = q[0], x1 = q[1];
+          v[gi * 8 + 0] = x0.x; v[gi * 8 + 1] = x0.y; v[gi * 8 + 2] = x0.z; v[gi * 8 + 3] = x0.w;
+          v[gi * 8 + 4] = x1.x; v[gi * 8 + 5] = x1.y; v[gi * 8 + 6] = x1.z; v[gi * 8 + 7] = x1.w;
+        } else {
 #pragma unroll
-      for (int rh = 0; rh < (R >> (bl + 1)); rh++) {
-        uint32_t p0 = pbase | ((uint32_t)(rh << (bl + 1)) << (g.t + lo));
-        uint32_t tw_v = twiddle<INV>(tw, g.log_n, layer, g.h_of(p0, b));
-#pragma unroll
-        for (int rl = 0; rl < (1 << bl); rl++) {
-          const int e0 = gi * R + (rh << (bl + 1)) + rl;
-          const int e1 = e0 + (1 << bl);
-          uint32_t x0 = v[e0], x1 = v[e1];
-          if (INV) {
-            v[e0] = add(x0, x1);
-            v[e1] = mul(sub(x0, x1), tw_v);
-          } else {
-            uint32_t tmp = mul(x1, tw_v);
-            v[e0] = add(x0, tmp);
-            v[e1] = sub(x0, tmp);
-          }
+          for (int r = 0; r < 8; r++) v[gi * 8 + r] = src[(i0 + r) & smask];
         }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) v[gi * R + r] = src[index_of(pb | (r << (T + LO))) & smask];
       }
     }
   }
-}
 
-template <int B, int NT, int W>
-__device__ __forceinline__ void lds_write(uint32_t* s, const uint32_t (&v)[(1 << B) / NT], const Geo& g,
-                                          uint32_t lo) {
-  constexpr int E = Tile<B, NT>::E;
-  constexpr int R = 1 << W;
+  template <int LO, int W>
+  __device__ __forceinline__ void store(uint32_t (&v)[E]) const {
+    constexpr int R = 1 << W;
+    if (A.scale) {
 #pragma unroll
-  for (int gi = 0; gi < E / R; gi++) {
-    uint32_t pbase = spread(threadIdx.x + NT * gi, g.t + lo, W);
+      for (int e = 0; e < E; e++) v[e] = mul(v[e], A.scale);
+    }
 #pragma unroll
-    for (int r = 0; r < R; r++) s[pbase | ((uint32_t)r << (g.t + lo))] = v[gi * R + r];
-  }
-}
-template <int B, int NT, int W>
-__device__ __forceinline__ void lds_read(const uint32_t* s, uint32_t (&v)[(1 << B) / NT], const Geo& g,
-                                         uint32_t lo) {
-  constexpr int E = Tile<B, NT>::E;
-  constexpr int R = 1 << W;
+    for (int gi = 0; gi < E / R; gi++) {
+      const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+      if constexpr (KIND == MULTI) {
+        const uint32_t col = col_of(pb);
+        if (col < A.n_cols) {
+          SH_GLOBAL uint32_t* c = as_global(A.dst[col]);
 #pragma unroll
-  for (int gi = 0; gi < E / R; gi++) {
-    uint32_t pbase = spread(threadIdx.x + NT * gi, g.t + lo, W);
+          for (int r = 0; r < R; r++) c[(pb + (r << LO)) & ((1u << K) - 1)] = v[gi * R + r];
+        }
+      } else if constexpr (LO == 0 && T == 0 && W == 3) {
+        SH_GLOBAL uint4* q = reinterpret_cast<SH_GLOBAL uint4*>(dst + index_of(pb));
+        q[0] = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
+        q[1] = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
+      } else {
 #pragma unroll
-    for (int r = 0; r < R; r++) v[gi * R + r] = s[pbase | ((uint32_t)r << (g.t + lo))];
-  }
-}
-
-template <int B, int NT, int W>
-__device__ __forceinline__ void gmem_load(uint32_t (&v)[(1 << B) / NT], const Geo& g, const PassArgs& A,
-                                          uint32_t lo) {
-  constexpr int E = Tile<B, NT>::E;
-  constexpr int R = 1 << W;
-  const uint32_t smask = (1u << A.src_log) - 1;
-  const uint32_t* col0 = g.multi ? nullptr : A.src[g.col_group];
-#pragma unroll
-  for (int gi = 0; gi < E / R; gi++) {
-    uint32_t pbase = spread(threadIdx.x + NT * gi, g.t + lo, W);
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      uint32_t col, idx;
-      g.locate(pbase | ((uint32_t)r << (g.t + lo)), col, idx);
-      const uint32_t* c = g.multi ? (col < A.n_cols ? A.src[col] : nullptr) : col0;
-      v[gi * R + r] = c ? c[idx & smask] : 0u;
+        for (int r = 0; r < R; r++) dst[index_of(pb | (r << (T + LO)))] = v[gi * R + r];
+      }
     }
   }
-}
-template <int B, int NT, int W>
-__device__ __forceinline__ void gmem_store(const uint32_t (&v)[(1 << B) / NT], const Geo& g, const PassArgs& A,
-                                           uint32_t lo) {
-  constexpr int E = Tile<B, NT>::E;
-  constexpr int R = 1 << W;
-  uint32_t* col0 = g.multi ? nullptr : A.dst[g.col_group];
+
+  template <int LO, int W>
+  __device__ __forceinline__ void lds_write(const uint32_t (&v)[E]) const {
+    constexpr int R = 1 << W;
 #pragma unroll
-  for (int gi = 0; gi < E / R; gi++) {
-    uint32_t pbase = spread(threadIdx.x + NT * gi, g.t + lo, W);
+    for (int gi = 0; gi < E / R; gi++) {
+      const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      uint32_t col, idx;
-      g.locate(pbase | ((uint32_t)r << (g.t + lo)), col, idx);
-      uint32_t* c = g.multi ? (col < A.n_cols ? A.dst[col] : nullptr) : col0;
-      uint32_t x = v[gi * R + r];
-      if (A.scale) x = mul(x, A.scale);
-      if (c) c[idx] = x;
+      for (int r = 0; r < R; r++) s[pb | (r << (T + LO))] = v[gi * R + r];
     }
   }
-}
+  template <int LO, int W>
+  __device__ __forceinline__ void lds_read(uint32_t (&v)[E]) const {
+    constexpr int R = 1 << W;
+#pragma unroll
+    for (int gi = 0; gi < E / R; gi++) {
+      const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+#pragma unroll
+      for (int r = 0; r < R; r++) v[gi * R + r] = s[pb | (r << (T + LO))];
+    }
+  }
 
-// Dispatch on the (runtime) width of a stage: 3 except possibly the last chunk.
-#define SH_W_DISPATCH(w, CALL) \
-  do {                         \
-    if ((w) == 3) {            \
-      constexpr int W = 3;     \
-      CALL;                    \
-    } else if ((w) == 2) {     \
-      constexpr int W = 2;     \
-      CALL;                    \
-    } else {                   \
-      constexpr int W = 1;     \
-      CALL;                    \
-    }                          \
-  } while (0)
+  // radix-2^W butterflies on layer bits [LO, LO + W); forward passes skip layers >= act_log
+  template <int LO, int W>
+  __device__ __forceinline__ void stage(uint32_t (&v)[E]) const {
+    constexpr int R = 1 << W;
+    static_for<0, W>([&](auto si) {
+      constexpr int BL = INV ? si.value : (W - 1 - si.value);
+      constexpr int BIT = LO + BL;
+      if (!INV && A.a + BIT >= A.act_log) return;  // uniform
+      const SH_GLOBAL uint32_t* lvl = level(BIT);
+#pragma unroll
+      for (int gi = 0; gi < E / R; gi++) {
+        const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+#pragma unroll
+        for (int rh = 0; rh < (R >> (BL + 1)); rh++) {
+          const uint32_t tw = twiddle<BIT>(lvl, pb | ((uint32_t)(rh << (BL + 1)) << (T + LO)));
+#pragma unroll
+          for (int rl = 0; rl < (1 << BL); rl++) {
+            const int e0 = gi * R + (rh << (BL + 1)) + rl;
+            const int e1 = e0 + (1 << BL);
+            const uint32_t x0 = v[e0], x1 = v[e1];
+            if constexpr (INV) {
+              v[e0] = add(x0, x1);
+              v[e1] = mul(sub(x0, x1), tw);
+            } else {
+              const uint32_t t = mul(x1, tw);
+              v[e0] = add(x0, t);
+              v[e1] = sub(x0, t);
+            }
+          }
+        }
+      }
+    });
+  }
 
-template <bool INV, int B, int NT>
+  __device__ __forceinline__ void run() {
+    uint32_t v[E];
+    static_for<0, NCH>([&](auto it) {
+      constexpr int c = INV ? it.value : NCH - 1 - it.value;
+      constexpr int LO = 3 * c;
+      constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
+      if constexpr (it.value == 0) {
+        load<LO, W>(v);
+      } else {
+        lds_read<LO, W>(v);
+      }
+      stage<LO, W>(v);
+      if constexpr (it.value + 1 < NCH) {
+        if constexpr (it.value > 0) __syncthreads();  // previous round's reads are done
+        lds_write<LO, W>(v);
+        __syncthreads();
+      } else {
+        store<LO, W>(v);
+      }
+    });
+  }
+};
+
+template <bool INV, int B, int NT, int K, int KIND>
 __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs A) {
-  constexpr int E = Tile<B, NT>::E;
   __shared__ uint32_t s[1 << B];
-  uint32_t v[E];
-
-  Geo g;
-  g.t = A.t;
-  g.k = A.k;
-  g.hb = A.hb;
-  g.a = A.a;
-  g.log_n = A.log_n;
-  g.multi = A.multi;
-  g.col_group = blockIdx.x % A.n_groups;
-  uint32_t tile = blockIdx.x / A.n_groups;
-  g.lob = tile & ((1u << A.lob_count_log) - 1);
-  g.hi_tile = tile >> A.lob_count_log;
-
-  // layer chunks of width <= 3 over [0, k); inverse runs them upward, forward downward
-  const uint32_t n_chunks = (A.k + 2) / 3;
-  const uint32_t limit = INV ? A.k : (A.act_log > A.a ? std::min(A.k, A.act_log - A.a) : 0u);
-  auto chunk_lo = [&](uint32_t c) { return 3 * c; };
-  auto chunk_w = [&](uint32_t c) { return std::min(3u, A.k - 3 * c); };
-
-  uint32_t c = INV ? 0 : n_chunks - 1;
-  {
-    uint32_t lo = chunk_lo(c), w = chunk_w(c);
-    SH_W_DISPATCH(w, (gmem_load<B, NT, W>(v, g, A, lo)));
-  }
-  for (uint32_t it = 0; it < n_chunks; it++) {
-    uint32_t lo = chunk_lo(c), w = chunk_w(c);
-    SH_W_DISPATCH(w, (stage<INV, B, NT, W>(v, g, A.tw, lo, limit)));
-    if (it + 1 == n_chunks) break;
-    uint32_t cn = INV ? c + 1 : c - 1;
-    uint32_t lon = chunk_lo(cn), wn = chunk_w(cn);
-    if (it) __syncthreads();
-    SH_W_DISPATCH(w, (lds_write<B, NT, W>(s, v, g, lo)));
-    __syncthreads();
-    SH_W_DISPATCH(wn, (lds_read<B, NT, W>(s, v, g, lon)));
-    c = cn;
-  }
-  {
-    uint32_t lo = chunk_lo(c), w = chunk_w(c);
-    SH_W_DISPATCH(w, (gmem_store<B, NT, W>(v, g, A, lo)));
-  }
+  Pass<INV, B, NT, K, KIND> ps(A, s);
+  ps.run();
 }
 
 // ---- circle domains of log size 1 and 2 (stwo CpuBackend special cases) ----
@@ -319,14 +339,12 @@ __global__ void fft_tiny_kernel(uint32_t* const* dst, const uint32_t* const* src
 constexpr int BS = 14;  // strided tile bits (64 KiB of LDS)
 constexpr uint32_t KS_MAX = BS - 5;
 
-static int threads_for(int B) { return B == 12 ? 256 : 512; }
-
-struct Pass {
+struct PassPlan {
   uint32_t a, k, t, hb, multi, B;
 };
 
-static std::vector<Pass> plan(uint32_t log_n) {
-  std::vector<Pass> ps;
+static std::vector<PassPlan> plan(uint32_t log_n) {
+  std::vector<PassPlan> ps;
   if (log_n <= 12) {
     ps.push_back({0, log_n, 0, 12 - log_n, 1, 12});
     return ps;
@@ -350,14 +368,34 @@ static std::vector<Pass> plan(uint32_t log_n) {
   return ps;
 }
 
-static const char* pass_name(bool inv, const Pass& p) {
+static const char* pass_name(bool inv, const PassPlan& p) {
   if (p.multi) return inv ? "fft_inv_tile" : "fft_fwd_tile";
   if (p.a == 0) return inv ? "fft_inv_contig" : "fft_fwd_contig";
   return inv ? "fft_inv_strided" : "fft_fwd_strided";
 }
 
+template <bool INV, int B, int NT, int K, int KIND>
+static void launch_k(uint32_t blocks, const PassArgs& A) {
+  fft_pass_kernel<INV, B, NT, K, KIND><<<blocks, NT, 0, ctx().stream>>>(A);
+}
+
+template <bool INV, int K>
+static void launch_multi(uint32_t blocks, const PassArgs& A, uint32_t k) {
+  if constexpr (K >= 3) {
+    if (k == K) return launch_k<INV, 12, 256, K, MULTI>(blocks, A);
+    launch_multi<INV, K - 1>(blocks, A, k);
+  }
+}
+template <bool INV, int K>
+static void launch_strided(uint32_t blocks, const PassArgs& A, uint32_t k) {
+  if constexpr (K >= 1) {
+    if (k == K) return launch_k<INV, BS, 512, K, STRIDED>(blocks, A);
+    launch_strided<INV, K - 1>(blocks, A, k);
+  }
+}
+
 template <bool INV>
-static void launch_pass(const Pass& p, const PassArgs& base) {
+static void launch_pass(const PassPlan& p, const PassArgs& base) {
   const char* name = pass_name(INV, p);
   PassArgs A = base;
   A.a = p.a;
@@ -366,6 +404,7 @@ static void launch_pass(const Pass& p, const PassArgs& base) {
   A.hb = p.hb;
   A.multi = p.multi;
   uint32_t blocks;
+  require(p.multi || p.a == 0 || (p.t == BS - p.k && p.hb == 0), "fft: unsupported pass shape");
   if (p.multi) {
     A.n_groups = ceil_div(A.n_cols, 1u << p.hb);
     A.lob_count_log = 0;
@@ -376,14 +415,15 @@ static void launch_pass(const Pass& p, const PassArgs& base) {
     uint32_t hi_log = A.log_n - p.a - p.k - p.hb;
     blocks = A.n_cols << (A.lob_count_log + hi_log);
   }
-  hipStream_t st = ctx().stream;
   timed(name, [&] {
-    if (p.B == 12)
-      fft_pass_kernel<INV, 12, 256><<<blocks, 256, 0, st>>>(A);
-    else if (p.B == 13)
-      fft_pass_kernel<INV, 13, 512><<<blocks, 512, 0, st>>>(A);
+    if (p.multi)
+      launch_multi<INV, 12>(blocks, A, p.k);
+    else if (p.a == 0 && p.B == 12)
+      launch_k<INV, 12, 256, 12, CONTIG>(blocks, A);
+    else if (p.a == 0)
+      launch_k<INV, 13, 512, 13, CONTIG>(blocks, A);
     else
-      fft_pass_kernel<INV, BS, 512><<<blocks, 512, 0, st>>>(A);
+      launch_strided<INV, KS_MAX>(blocks, A, p.k);
   });
 }
 

@@ -172,6 +172,13 @@ __global__ void fill_uniform_kernel(uint32_t* dst, uint64_t n, uint64_t seed, ui
   }
 }
 
+// plain 16-byte copy (column copies; also the reference kernel for HBM counter calibration)
+__global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace sh
 
 using namespace sh;
@@ -223,6 +230,17 @@ extern "C" int stwo_hip_fill_uniform_m31(uint32_t* dst, uint64_t n, uint64_t see
     uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 256u * 64u);
     timed("fill_uniform", [&] {
       fill_uniform_kernel<<<blocks, 256, 0, ctx().stream>>>(dst, n, seed, offset);
+    });
+  });
+}
+extern "C" int stwo_hip_device_copy(void* dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    require(((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0, "stwo_hip_device_copy: needs 16-byte alignment");
+    uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 256u * 32u);
+    timed("copy16", [&] {
+      copy16_kernel<<<blocks, 256, 0, ctx().stream>>>((const uint4*)src, (uint4*)dst, n16);
     });
   });
 }

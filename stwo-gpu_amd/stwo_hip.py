@@ -101,6 +101,7 @@ def _declare(L):
         "stwo_hip_memcpy": (C.c_int, [vp, vp, sz]),
         "stwo_hip_release_cached_memory": (C.c_int, []),
         "stwo_hip_fill_uniform_m31": (C.c_int, [vp, u64, u64, u64]),
+        "stwo_hip_device_copy": (C.c_int, [vp, vp, sz]),
         "stwo_hip_twiddles": (C.c_int, [u32, u32, u32, C.POINTER(vp), C.POINTER(vp)]),
         "stwo_hip_release_twiddles": (C.c_int, []),
         "stwo_hip_interpolate_columns_async": (C.c_int, [pp, u32p, u32, vp, u64]),
@@ -485,6 +486,10 @@ def cached_twiddles(coset: Coset):
     tw, itw = vp(), vp()
     _c(lib().stwo_hip_twiddles(coset.initial_index, coset.step_size, coset.log_size, C.byref(tw), C.byref(itw)))
     return tw.value, itw.value
+
+
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes=None):
+    _c(lib().stwo_hip_device_copy(dst.ptr, src.ptr, src.nbytes if nbytes is None else nbytes))
 
 
 def fill_uniform(col: BaseFieldVec, seed, offset=0):
