@@ -34,8 +34,7 @@ def algorithmic_bytes(C, n):
     m = 2 * n  # LDE length
     return {
         "fft_inv_contig": 8 * C * n,           # read evaluations, write partial IFFT
-        "fft_inv_strided": 8 * C * n,          # read, write coefficients
-        "fft_fwd_strided": 4 * C * n + 4 * C * m,  # read coefficients, write LDE (partial)
+        "fft_lde_strided": 4 * C * n + 4 * C * n + 4 * C * m,  # read partial IFFT, write coefficients + partial LDE
         "fft_fwd_contig": 8 * C * m,           # read, write LDE
         "merkle_leaf": 4 * C * m + 32 * m,     # read every LDE value, write 2^21 leaf hashes
     }
@@ -105,8 +104,7 @@ def main():
         del a, b
 
     def step():
-        S.interpolate_columns_oop_async(evals, coeffs, [LOG_N] * C, itw, tree_len)
-        S.evaluate_columns_async(coeffs, [LOG_N] * C, ldes, [LOG_N + LOG_BLOWUP] * C, tw, tree_len)
+        S.lde_columns_async(evals, coeffs, ldes, [LOG_N] * C, LOG_BLOWUP, tw, itw, tree_len)
         tree.commit_async()
 
     for _ in range(args.warmup):
@@ -131,8 +129,7 @@ def main():
         step()
     S.synchronize()
     S.kernel_timing(False)
-    names = ["fft_inv_contig", "fft_inv_strided", "fft_fwd_strided", "fft_fwd_contig", "merkle_leaf",
-             "merkle_subtree"]
+    names = ["fft_inv_contig", "fft_lde_strided", "fft_fwd_contig", "merkle_leaf", "merkle_subtree"]
     stats = {k: S.kernel_stats(k) for k in names}
     alg = algorithmic_bytes(C, n)
     kernels = {}

@@ -206,6 +206,13 @@ int stwo_hip_release_twiddles(void);
  *             (dst may alias coeffs only when eval_log == coeff_log). */
 int stwo_hip_interpolate_columns_async(uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols,
                                        const uint32_t *itwiddles, uint64_t tree_len);
+/* Low-degree extension in one call: evals[i] (2^log_sizes[i] evaluations) are interpolated
+ * into coeffs[i] (may alias evals[i]) and evaluated on the canonic domain of log
+ * log_sizes[i] + log_blowup into lde[i].  Columns of log 14..21 take a fused path (three
+ * passes over HBM instead of four). */
+int stwo_hip_lde_columns_async(const uint32_t *const *evals, uint32_t *const *coeffs, uint32_t *const *lde,
+                               const uint32_t *log_sizes, uint32_t n_cols, uint32_t log_blowup,
+                               const uint32_t *twiddles, const uint32_t *itwiddles, uint64_t tree_len);
 /* Out-of-place interpolate: src[i] (evaluations) is read, dst[i] receives the coefficients. */
 int stwo_hip_interpolate_columns_oop_async(const uint32_t *const *src, uint32_t *const *dst,
                                            const uint32_t *log_sizes, uint32_t n_cols,

@@ -79,7 +79,7 @@ __device__ __forceinline__ uint32_t spread(uint32_t gamma) {
 //   MULTI   (a = 0, log_n = K): T = 0, HB = B - K bits selecting the column
 enum Kind { CONTIG = 0, STRIDED = 1, MULTI = 2 };
 
-template <bool INV, int B, int NT, int K, int KIND>
+template <bool INV, int B, int NT, int K, int KIND, bool MASKED = false>
 struct Pass {
   static constexpr int T = KIND == STRIDED ? B - K : 0;
   static constexpr int HB = KIND == MULTI ? B - K : 0;
@@ -88,7 +88,7 @@ struct Pass {
 
   const PassArgs& A;
   uint32_t* s;
-  uint32_t tile, col_group;
+  uint32_t tile, col_group, lob;
   uint64_t block_base;  // index of tile position 0 (CONTIG / STRIDED)
   uint32_t hi_tile;     // tile's high index bits (twiddle block index)
   const SH_GLOBAL uint32_t* src;  // CONTIG / STRIDED: this block's column
@@ -100,15 +100,20 @@ struct Pass {
     if constexpr (KIND == MULTI) {
       block_base = 0;
       hi_tile = 0;
+      lob = 0;
       src = nullptr;
       dst = nullptr;
     } else {
-      const uint32_t lob = tile & ((1u << A.lob_count_log) - 1);
-      hi_tile = tile >> A.lob_count_log;
-      block_base = ((uint64_t)hi_tile << (A.a + K)) | ((uint64_t)lob << T);
+      lob = tile & ((1u << A.lob_count_log) - 1);
+      set_hi(tile >> A.lob_count_log);
       src = as_global(A.src[col_group]);
       dst = as_global(A.dst[col_group]);
     }
+  }
+  // place the tile in high block `hi` (the bits above a + K) of its column
+  __device__ __forceinline__ void set_hi(uint32_t hi) {
+    hi_tile = hi;
+    block_base = ((uint64_t)hi_tile << (A.a + K)) | ((uint64_t)lob << T);
   }
 
   // global index (within its column) of tile position p
@@ -199,6 +204,9 @@ struct Pass {
     }
   }
 
+  // LDS exchanges use the plain position as address.  Chunks 0 and 1 of the contiguous
+  // passes are 8-way bank conflicted; an XOR swizzle that removes every conflict costs
+  // ~150 VALU instructions per thread and measured slower (these passes are VALU-bound).
   template <int LO, int W>
   __device__ __forceinline__ void lds_write(const uint32_t (&v)[E]) const {
     constexpr int R = 1 << W;
@@ -227,7 +235,9 @@ struct Pass {
     static_for<0, W>([&](auto si) {
       constexpr int BL = INV ? si.value : (W - 1 - si.value);
       constexpr int BIT = LO + BL;
-      if (!INV && A.a + BIT >= A.act_log) return;  // uniform
+      if constexpr (MASKED) {
+        if (A.a + BIT >= A.act_log) return;  // uniform: layer over zero padding
+      }
       const SH_GLOBAL uint32_t* lvl = level(BIT);
 #pragma unroll
       for (int gi = 0; gi < E / R; gi++) {
@@ -277,11 +287,76 @@ struct Pass {
   }
 };
 
-template <bool INV, int B, int NT, int K, int KIND>
+template <bool INV, int B, int NT, int K, int KIND, bool MASKED>
 __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs A) {
   __shared__ uint32_t s[1 << B];
-  Pass<INV, B, NT, K, KIND> ps(A, s);
+  Pass<INV, B, NT, K, KIND, MASKED> ps(A, s);
   ps.run();
+}
+
+// Low-degree extension middle pass.  For columns of log n (14 <= n <= 21), after the
+// contiguous inverse pass (layers 0..11): inverse layers 12..n-1 on a strided tile,
+// scale by 2^-n and store the coefficients; then, from the same registers (the
+// inverse's last chunk mapping is the forward's first), forward layers n-1..12 of the
+// evaluation domain (log m = n + blowup) for each of its 2^(m-n) blocks -- the top
+// m-n forward layers only duplicate the zero-padded coefficients -- and store the
+// partial LDE.  The forward contiguous pass (layers 0..11) then finishes the LDE.
+// One HBM read of the evaluations' partial transform replaces the separate inverse
+// and forward strided passes (which read the coefficients back).
+constexpr int BS_LDE = 14;
+// The fused middle pass saves one HBM read of the coefficients, but it holds the
+// coefficients and the forward tile in registers (4 waves/SIMD instead of 8); on gfx950
+// the separate strided passes measured faster (1.63 ms vs 1.73 ms for 256 columns of log
+// 20), so lde_same_size uses them.
+constexpr bool kFusedLde = false;
+constexpr int NT_S = 1024;  // threads per 2^14-element tile: 16 values per thread
+template <int K>
+__global__ __launch_bounds__(NT_S) void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
+  __shared__ uint32_t s[1 << BS_LDE];
+  using PI = Pass<true, BS_LDE, NT_S, K, STRIDED>;
+  using PF = Pass<false, BS_LDE, NT_S, K, STRIDED, false>;
+  constexpr int E = PI::E, NCH = PI::NCH;
+  constexpr int LO_TOP = 3 * (NCH - 1), W_TOP = K - LO_TOP;
+  PI pi(Ai, s);
+  uint32_t v[E];
+  static_for<0, NCH>([&](auto it) {
+    constexpr int c = it.value;
+    constexpr int LO = 3 * c;
+    constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
+    if constexpr (c == 0) {
+      pi.template load<LO, W>(v);
+    } else {
+      pi.template lds_read<LO, W>(v);
+    }
+    pi.template stage<LO, W>(v);
+    if constexpr (c + 1 < NCH) {
+      if constexpr (c > 0) __syncthreads();
+      pi.template lds_write<LO, W>(v);
+      __syncthreads();
+    }
+  });
+  pi.template store<LO_TOP, W_TOP>(v);  // scales v in place: v now holds the coefficients
+  PF pf(Af, s);
+  for (uint32_t b = 0; b < n_blocks; b++) {
+    pf.set_hi(b);
+    uint32_t w[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) w[e] = v[e];
+    static_for<0, NCH>([&](auto it) {
+      constexpr int c = NCH - 1 - it.value;
+      constexpr int LO = 3 * c;
+      constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
+      if constexpr (it.value > 0) pf.template lds_read<LO, W>(w);
+      pf.template stage<LO, W>(w);
+      if constexpr (c > 0) {
+        __syncthreads();  // earlier readers of s are done
+        pf.template lds_write<LO, W>(w);
+        __syncthreads();
+      } else {
+        pf.template store<LO, W>(w);
+      }
+    });
+  }
 }
 
 // ---- circle domains of log size 1 and 2 (stwo CpuBackend special cases) ----
@@ -376,7 +451,11 @@ static const char* pass_name(bool inv, const PassPlan& p) {
 
 template <bool INV, int B, int NT, int K, int KIND>
 static void launch_k(uint32_t blocks, const PassArgs& A) {
-  fft_pass_kernel<INV, B, NT, K, KIND><<<blocks, NT, 0, ctx().stream>>>(A);
+  // forward passes that reach layers at or above act_log (zero-padded coefficients) skip them
+  if (!INV && A.act_log < A.a + K)
+    fft_pass_kernel<false, B, NT, K, KIND, true><<<blocks, NT, 0, ctx().stream>>>(A);
+  else
+    fft_pass_kernel<INV, B, NT, K, KIND, false><<<blocks, NT, 0, ctx().stream>>>(A);
 }
 
 template <bool INV, int K>
@@ -389,7 +468,7 @@ static void launch_multi(uint32_t blocks, const PassArgs& A, uint32_t k) {
 template <bool INV, int K>
 static void launch_strided(uint32_t blocks, const PassArgs& A, uint32_t k) {
   if constexpr (K >= 1) {
-    if (k == K) return launch_k<INV, BS, 512, K, STRIDED>(blocks, A);
+    if (k == K) return launch_k<INV, BS, NT_S, K, STRIDED>(blocks, A);
     launch_strided<INV, K - 1>(blocks, A, k);
   }
 }
@@ -481,6 +560,96 @@ void evaluate_same_size(uint32_t* const* d_dst, const uint32_t* const* d_src, ui
     A.src_log = first ? src_log : log_n;
     A.act_log = src_log;
     launch_pass<false>(ps[i], A);
+  }
+}
+
+template <int K>
+static void launch_lde_strided(uint32_t blocks, const PassArgs& Ai, const PassArgs& Af, uint32_t n_blocks, uint32_t k) {
+  if constexpr (K >= 2) {
+    if (k == K) {
+      fft_lde_strided_kernel<K><<<blocks, NT_S, 0, ctx().stream>>>(Ai, Af, n_blocks);
+      return;
+    }
+    launch_lde_strided<K - 1>(blocks, Ai, Af, n_blocks, k);
+  }
+}
+
+// Interpolate d_src (2^n evaluations) into d_coeffs and evaluate on the canonic domain of
+// log n + log_blowup into d_lde.  d_src may equal d_coeffs.
+void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint32_t* const* d_lde, uint32_t n_cols,
+                   uint32_t n, uint32_t log_blowup, const uint32_t* tw, const uint32_t* itw, uint64_t tree_len) {
+  const uint32_t m = n + log_blowup;
+  require(m <= 30, "lde: log size out of range");
+  require(tree_len >= (1ull << (m - 1)), "lde: twiddle tree smaller than the evaluation domain");
+  if (!kFusedLde || n < 14 || n > 12 + KS_MAX) {
+    interpolate_same_size(d_coeffs, n_cols, n, itw, tree_len, d_src);
+    evaluate_same_size(d_lde, (const uint32_t* const*)d_coeffs, n_cols, n, m, tw, tree_len);
+    return;
+  }
+  const uint32_t k = n - 12, t = BS_LDE - k;
+  PassArgs Ai{};
+  Ai.src = d_src;
+  Ai.dst = d_coeffs;
+  Ai.tw = itw + (tree_len - (1ull << (n - 1)));
+  Ai.n_cols = n_cols;
+  Ai.log_n = n;
+  Ai.src_log = n;
+  Ai.act_log = n;
+  // inverse layers 0..11, evaluations -> d_coeffs
+  launch_pass<true>(PassPlan{0, 12, 0, 0, 0, 12}, Ai);
+  // fused middle pass
+  Ai.src = (const uint32_t* const*)d_coeffs;
+  Ai.a = 12;
+  Ai.k = k;
+  Ai.t = t;
+  Ai.hb = 0;
+  Ai.multi = 0;
+  Ai.scale = (1u << (31 - n)) % P;
+  Ai.n_groups = n_cols;
+  Ai.lob_count_log = 12 - t;
+  PassArgs Af = Ai;
+  Af.src = (const uint32_t* const*)d_lde;
+  Af.dst = d_lde;
+  Af.tw = tw + (tree_len - (1ull << (m - 1)));
+  Af.log_n = m;
+  Af.src_log = m;
+  Af.act_log = n;
+  Af.scale = 0;
+  const uint32_t blocks = n_cols << (12 - t);
+  timed("fft_lde_strided", [&] { launch_lde_strided<KS_MAX>(blocks, Ai, Af, 1u << log_blowup, k); });
+  // forward layers 0..11 on the LDE
+  PassArgs Ac{};
+  Ac.src = (const uint32_t* const*)d_lde;
+  Ac.dst = d_lde;
+  Ac.tw = tw + (tree_len - (1ull << (m - 1)));
+  Ac.n_cols = n_cols;
+  Ac.log_n = m;
+  Ac.src_log = m;
+  Ac.act_log = m;
+  launch_pass<false>(PassPlan{0, 12, 0, 0, 0, 12}, Ac);
+}
+
+void lde_mixed(const uint32_t* const* src, uint32_t* const* coeffs, uint32_t* const* lde, const uint32_t* logs,
+               uint32_t n_cols, uint32_t log_blowup, const uint32_t* tw, const uint32_t* itw, uint64_t tree_len) {
+  std::map<uint32_t, std::vector<const void*>> by;
+  for (uint32_t i = 0; i < n_cols; i++) {
+    auto& e = by[logs[i]];
+    e.push_back(src[i]);
+    e.push_back(coeffs[i]);
+    e.push_back(lde[i]);
+  }
+  for (auto& kv : by) {
+    size_t cnt = kv.second.size() / 3;
+    std::vector<const void*> t(3 * cnt);
+    for (size_t i = 0; i < cnt; i++) {
+      t[i] = kv.second[3 * i];
+      t[cnt + i] = kv.second[3 * i + 1];
+      t[2 * cnt + i] = kv.second[3 * i + 2];
+    }
+    DevBuf d = upload(t.data(), t.size() * sizeof(void*));
+    const void* const* base = d.as<const void*>();
+    lde_same_size((const uint32_t* const*)base, (uint32_t* const*)(base + cnt), (uint32_t* const*)(base + 2 * cnt),
+                  (uint32_t)cnt, kv.first, log_blowup, tw, itw, tree_len);
   }
 }
 
@@ -587,6 +756,19 @@ extern "C" int stwo_hip_interpolate_columns_oop_async(const uint32_t* const* src
                                                       const uint32_t* log_sizes, uint32_t n_cols,
                                                       const uint32_t* itwiddles, uint64_t tree_len) {
   return guarded([&] { fft::interpolate_mixed(dst, log_sizes, n_cols, itwiddles, tree_len, src); });
+}
+
+extern "C" int stwo_hip_lde_columns_async(const uint32_t* const* evals, uint32_t* const* coeffs, uint32_t* const* lde,
+                                          const uint32_t* log_sizes, uint32_t n_cols, uint32_t log_blowup,
+                                          const uint32_t* twiddles, const uint32_t* itwiddles, uint64_t tree_len) {
+  return guarded([&] {
+    for (uint32_t i = 0; i < n_cols; i++) {
+      require(log_sizes[i] >= 1, "lde: empty column");
+      require((const void*)lde[i] != (const void*)coeffs[i] && (const void*)lde[i] != (const void*)evals[i],
+              "lde: the LDE buffer must not alias its inputs");
+    }
+    fft::lde_mixed(evals, coeffs, lde, log_sizes, n_cols, log_blowup, twiddles, itwiddles, tree_len);
+  });
 }
 
 extern "C" int stwo_hip_evaluate_columns_async(const uint32_t* const* coeffs, const uint32_t* coeff_log,

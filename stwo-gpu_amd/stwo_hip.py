@@ -107,6 +107,7 @@ def _declare(L):
         "stwo_hip_interpolate_columns_async": (C.c_int, [pp, u32p, u32, vp, u64]),
         "stwo_hip_evaluate_columns_async": (C.c_int, [pp, u32p, pp, u32p, u32, vp, u64]),
         "stwo_hip_interpolate_columns_oop_async": (C.c_int, [pp, pp, u32p, u32, vp, u64]),
+        "stwo_hip_lde_columns_async": (C.c_int, [pp, pp, pp, u32p, u32, u32, vp, vp, u64]),
         "stwo_hip_merkle_commit_async": (C.c_int, [u32, pp, u32p, pp, u32]),
         "stwo_hip_merkle_root": (C.c_int, [vp, vp]),
         "stwo_hip_timer_begin": (C.c_int, []),
@@ -506,6 +507,13 @@ def interpolate_columns_oop_async(src, dst, log_sizes, itw_ptr, tree_len):
     logs = np.ascontiguousarray(log_sizes, np.uint32)
     _c(lib().stwo_hip_interpolate_columns_oop_async(_ptr_array([c.ptr for c in src]), _ptr_array([c.ptr for c in dst]),
                                                     logs.ctypes.data_as(u32p), len(src), itw_ptr, tree_len))
+
+
+def lde_columns_async(evals, coeffs, ldes, log_sizes, log_blowup, tw_ptr, itw_ptr, tree_len):
+    logs = np.ascontiguousarray(log_sizes, np.uint32)
+    _c(lib().stwo_hip_lde_columns_async(_ptr_array([c.ptr for c in evals]), _ptr_array([c.ptr for c in coeffs]),
+                                        _ptr_array([c.ptr for c in ldes]), logs.ctypes.data_as(u32p), len(evals),
+                                        log_blowup, tw_ptr, itw_ptr, tree_len))
 
 
 def evaluate_columns_async(coeffs, coeff_logs, dst, eval_logs, tw_ptr, tree_len):

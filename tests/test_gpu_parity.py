@@ -400,3 +400,26 @@ def test_errors_are_reported_not_fatal():
     with pytest.raises(S.StwoHipError):
         S.bit_reverse_column(col)  # not a power of two
     S.bit_reverse_column(S.BaseFieldVec.from_numpy(np.arange(16, dtype=np.uint32)))  # still usable
+
+
+# ------------------------------------------------------------------ one-call LDE (fused middle pass)
+@pytest.mark.parametrize("log_n,blow,n_cols,in_place", [(3, 1, 3, False), (13, 1, 2, False), (14, 1, 3, True),
+                                                         (15, 2, 2, False), (17, 3, 2, True), (20, 1, 2, False),
+                                                         (21, 1, 1, True), (22, 1, 1, False)])
+def test_lde_columns_fused(log_n, blow, n_cols, in_place):
+    m = log_n + blow
+    root = S.canonic_half_coset(m)
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << log_n, 7000 + 31 * log_n + c) for c in range(n_cols)]
+    src = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    coeffs = src if in_place else [S.BaseFieldVec.uninitialized(1 << log_n) for _ in range(n_cols)]
+    ldes = [S.BaseFieldVec.uninitialized(1 << m) for _ in range(n_cols)]
+    S.lde_columns_async(src, coeffs, ldes, [log_n] * n_cols, blow, tw, itw, 1 << root.log_size)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    oc, ol = O.lde_columns(evals, blow, otw, oitw)
+    for c in range(n_cols):
+        assert (coeffs[c].to_numpy() == oc[c]).all()
+        assert (ldes[c].to_numpy() == ol[c]).all()
+        if not in_place:
+            assert (src[c].to_numpy() == evals[c]).all()  # inputs untouched
