@@ -65,25 +65,10 @@ def cpu_baseline(sample_cols):
     return sample_cols * (n << LOG_BLOWUP) / dt, dt, threads, ldes[0], layers
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-sample-cols", type=int, default=32)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
-        sys.exit(2)
-
+def bench_lde_commit(args):
     import numpy as np
     import stwo_hip as S
 
-    S.check(S.lib().stwo_hip_init(0))
     C, n = N_COLS, 1 << LOG_N
     m = n << LOG_BLOWUP
     evals = [S.BaseFieldVec.uninitialized(n) for _ in range(C)]
@@ -96,12 +81,6 @@ def main():
     tree_len = 1 << root.log_size
     tree = S.MerkleTree(ldes, [LOG_N + LOG_BLOWUP] * C)
     S.synchronize()
-    if args.calibrate_copy:
-        # copy the first 256 evaluation columns' worth of bytes: 1 GiB read + 1 GiB written
-        a, b = S.BaseFieldVec.uninitialized(1 << 28), S.BaseFieldVec.uninitialized(1 << 28)
-        S.device_copy(b, a)
-        S.synchronize()
-        del a, b
 
     def step():
         S.lde_columns_async(evals, coeffs, ldes, [LOG_N] * C, LOG_BLOWUP, tw, itw, tree_len)
@@ -183,7 +162,202 @@ def main():
                                "sample": f"{args.cpu_sample_cols} of the 256 columns (same seeds), full LDE + "
                                          f"Merkle tree over them; {dt:.2f} s",
                                "lde_column0_matches_gpu": parity}
-    print(json.dumps(out))
+    return out
+
+
+def _timed_steps(S, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    S.synchronize()
+    S.timer_begin()
+    for _ in range(steps):
+        step()
+    ms = S.timer_end()
+    return ms / steps
+
+
+def _kernel_avg(S, step, names, reps=3):
+    S.kernel_stats_reset()
+    S.kernel_timing(True)
+    for _ in range(reps):
+        step()
+    S.synchronize()
+    S.kernel_timing(False)
+    res = {}
+    for k in names:
+        cnt, tot = S.kernel_stats(k)
+        if cnt:
+            res[k] = {"launches_per_step": cnt / reps, "avg_ms": round(tot / cnt, 4)}
+    return res
+
+
+def _line(metric, value, unit, ms, args, config, roofline=None, extra=None, higher=True):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32 (M31/QM31)", "data": "synthetic (seeded uniform)", "config": config}
+    if roofline:
+        out["roofline"] = roofline
+    if extra:
+        out.update(extra)
+    return out
+
+
+def _roof(bytes_, avg_ms, kernel):
+    gbs = bytes_ / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes_per_launch": bytes_,
+            "avg_launch_ms": avg_ms}
+
+
+def bench_cfg1(args):
+    """16 columns of log 16: interpolate, evaluate on log 17, Merkle commit; root checked vs the oracle."""
+    import numpy as np
+    import stwo_hip as S
+
+    C, L = 16, 16
+    n, m = 1 << L, 1 << (L + 1)
+    evals = [S.BaseFieldVec.uninitialized(n) for _ in range(C)]
+    for c, col in enumerate(evals):
+        S.fill_uniform(col, 0x5EED0001, c * n)
+    coeffs = [S.BaseFieldVec.uninitialized(n) for _ in range(C)]
+    ldes = [S.BaseFieldVec.uninitialized(m) for _ in range(C)]
+    root = S.canonic_half_coset(L + 1)
+    tw, itw = S.cached_twiddles(root)
+    tree = S.MerkleTree(ldes, [L + 1] * C)
+
+    def step():
+        S.lde_columns_async(evals, coeffs, ldes, [L] * C, 1, tw, itw, 1 << root.log_size)
+        tree.commit_async()
+    ms = _timed_steps(S, step, args.steps, args.warmup)
+    extra = {"root": "".join(f"{w:08x}" for w in tree.root())}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        host = [O.fill_uniform_m31(n, 0x5EED0001, c * n) for c in range(C)]
+        otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+        t0 = time.perf_counter()
+        _, olde = O.lde_columns(host, 1, otw, oitw)
+        oroot = O.merkle_commit(olde)[0]
+        dt = time.perf_counter() - t0
+        extra["parity_root_matches_oracle"] = bool((oroot == tree.root()).all())
+        extra["cpu_baseline"] = {"value": C * m / dt, "unit": "elems/s", "kind": "port", "sample": "whole config",
+                                 "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    return _line("LDE+Merkle M31 elems/s (cfg1)", C * m / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "cfg1: 16 M31 columns log 16 -> LDE log 17 + Blake2s Merkle"}, extra=extra)
+
+
+def bench_batch_inverse(args):
+    """batch_inverse of 2^24 uniform QM31 elements (AoS)."""
+    import stwo_hip as S
+
+    n = 1 << 24
+    src, dst = S.SecureFieldVec.uninitialized(n), S.SecureFieldVec.uninitialized(n)
+    S.check(S.lib().stwo_hip_fill_uniform_m31(src.ptr, 4 * n, 0x5EED0002, 1 << 40))
+
+    def step():
+        S.lib().batch_inverse_secure_field(src.ptr, dst.ptr, n)
+    ms = _timed_steps(S, step, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["batch_inverse_qm31"])
+    avg = ks["batch_inverse_qm31"]["avg_ms"]
+    return _line("batch_inverse QM31 elems/s", n / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"}, _roof(32 * n, avg, "batch_inverse_qm31"),
+                 {"kernels": ks, "note": "ms_per_step is the synchronous ABI call (launch + sync)"})
+
+
+def bench_fri(args):
+    """fold_circle_into_line of a QM31 circle evaluation at log 22, then fold_line down to log 4."""
+    import numpy as np
+    import stwo_hip as S
+
+    L = 22
+    rng = np.random.default_rng(0x5EED0002)
+    src = S.SecureColumn.uninitialized(1 << L)
+    for k in range(4):
+        S.fill_uniform(src.columns[k], 0x5EED0002, (2 << 40) + k * (1 << L))
+    layers = [S.SecureColumn.zeros(1 << l) for l in range(L - 1, 3, -1)]
+    root = S.canonic_half_coset(L)  # the circle domain's half coset: its tree serves every line layer
+    tw, itw = S.cached_twiddles(root)
+    tree_len = 1 << root.log_size
+    alphas = [S.random_qm31(rng) for _ in range(len(layers))]
+
+    def step():
+        S.fold_circle_into_line_async(layers[0], src, alphas[0], itw, tree_len - len(layers[0]))
+        for i in range(1, len(layers)):
+            S.fold_line_async(layers[i - 1], layers[i], alphas[i], itw, tree_len - len(layers[i - 1]))
+    ms = _timed_steps(S, step, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["fold_circle_into_line", "fold_line"])
+    circle_bytes = 16 * (1 << L) + 2 * 16 * (1 << (L - 1)) + 4 * (1 << (L - 2))
+    line_bytes = sum(16 * (1 << l) + 16 * (1 << (l - 1)) + 4 * (1 << (l - 1)) for l in range(L - 1, 4, -1))
+    total = circle_bytes + line_bytes
+    return _line("FRI fold chain GB/s (log 22 circle -> line -> log 4)", total / (ms * 1e-3) / 1e9, "GB/s", ms, args,
+                 {"workload": "cfg2 FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
+                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line"),
+                 {"kernels": ks, "algorithmic_bytes_per_step": total})
+
+
+def bench_quotients(args):
+    """accumulate_quotients: 200 M31 columns at log 21, 3 random QM31 points, 1-3 samples per column."""
+    import numpy as np
+    import stwo_hip as S
+
+    L, C = 21, 200
+    n = 1 << L
+    rng = np.random.default_rng(0x5EED0003)
+    cols = [S.BaseFieldVec.uninitialized(n) for _ in range(C)]
+    for c, col in enumerate(cols):
+        S.fill_uniform(col, 0x5EED0003, c * n)
+    points = [S.random_circle_point_qm31(rng) for _ in range(3)]
+    per_point = [[] for _ in range(3)]
+    for c in range(C):
+        k = int(rng.integers(1, 4))
+        for pi in sorted(rng.choice(3, size=k, replace=False)):
+            per_point[pi].append((c, S.random_qm31(rng)))
+    batches = [(points[i], per_point[i]) for i in range(3) if per_point[i]]
+    samples = S.QuotientSamples(batches)
+    rc = S.random_qm31(rng)
+    out = S.SecureColumn.uninitialized(n)
+    dom = S.canonic_half_coset(L)
+
+    def step():
+        S.accumulate_quotients_async(dom, cols, rc, samples, out)
+    ms = _timed_steps(S, step, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["accumulate_quotients"])
+    alg = 4 * C * n + 16 * n
+    n_samples = sum(len(cv) for _, cv in batches)
+    return _line("accumulate_quotients rows/s", n / (ms * 1e-3), "rows/s", ms, args,
+                 {"workload": f"cfg3 quotients: {C} M31 columns log {L}, 3 QM31 points, {n_samples} samples"},
+                 _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients"), {"kernels": ks})
+
+
+CONFIGS = {"cfg2": bench_lde_commit, "cfg1": bench_cfg1, "cfg2-binv": bench_batch_inverse, "cfg2-fri": bench_fri,
+           "cfg3": bench_quotients}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--cpu-sample-cols", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this backend runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import stwo_hip as S
+
+    S.check(S.lib().stwo_hip_init(0))
+    if args.calibrate_copy:
+        # 1 GiB read + 1 GiB written by the library's plain copy kernel
+        a, b = S.BaseFieldVec.uninitialized(1 << 28), S.BaseFieldVec.uninitialized(1 << 28)
+        S.device_copy(b, a)
+        S.synchronize()
+        del a, b
+    print(json.dumps(CONFIGS[args.config](args)))
 
 
 if __name__ == "__main__":

@@ -229,6 +229,21 @@ int stwo_hip_merkle_commit_async(uint32_t n_cols, const uint32_t *const *cols, c
                                  StwoBlake2sHash *const *layers, uint32_t max_log);
 int stwo_hip_merkle_root(const StwoBlake2sHash *root_layer, StwoBlake2sHash *root_out);
 
+/* Asynchronous FRI folds (same arguments as fold_line / fold_circle_into_line). */
+int stwo_hip_fold_line_async(const uint32_t *itwiddles, size_t twiddle_offset, size_t n,
+                             uint32_t *const *eval_values, StwoQM31 alpha, uint32_t *const *folded_values);
+int stwo_hip_fold_circle_into_line_async(const uint32_t *itwiddles, size_t twiddle_offset, size_t n,
+                                         uint32_t *const *eval_values, StwoQM31 alpha,
+                                         uint32_t *const *folded_values);
+/* Asynchronous DEEP quotients.  columns: HOST array of n_columns device pointers (each of
+ * domain_size values); points/batch_sizes/column_indexes/values: HOST, in batch order;
+ * out: HOST array of 4 device columns. */
+int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_index, uint32_t half_coset_step_size,
+                                        uint32_t domain_size, const uint32_t *const *columns, uint32_t n_columns,
+                                        StwoQM31 random_coeff, uint32_t n_batches, const StwoCirclePointQM31 *points,
+                                        const uint32_t *batch_sizes, const uint32_t *column_indexes,
+                                        const StwoQM31 *values, uint32_t *const *out);
+
 /* Timing on the library stream (HIP events). */
 int stwo_hip_timer_begin(void);
 int stwo_hip_timer_end(float *elapsed_ms);

@@ -90,6 +90,23 @@ void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, siz
 
 using namespace sh;
 
+extern "C" int stwo_hip_fold_line_async(const uint32_t* itwiddles, size_t twiddle_offset, size_t n,
+                                        uint32_t* const* eval_values, StwoQM31 alpha, uint32_t* const* folded_values) {
+  return guarded([&] {
+    fold_line_async(itwiddles, twiddle_offset, n, eval_values, qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}},
+                    folded_values);
+  });
+}
+
+extern "C" int stwo_hip_fold_circle_into_line_async(const uint32_t* itwiddles, size_t twiddle_offset, size_t n,
+                                                    uint32_t* const* eval_values, StwoQM31 alpha,
+                                                    uint32_t* const* folded_values) {
+  return guarded([&] {
+    fold_circle_into_line_async(itwiddles, twiddle_offset, n, eval_values,
+                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values);
+  });
+}
+
 extern "C" void fold_line(const uint32_t* gpu_domain, size_t twiddle_offset, size_t n, uint32_t* const* eval_values,
                           StwoQM31 alpha, uint32_t* const* folded_values) {
   guarded([&] {

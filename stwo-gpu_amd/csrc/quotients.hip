@@ -142,6 +142,26 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
 
 using namespace sh;
 
+extern "C" int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_index, uint32_t half_coset_step_size,
+                                                   uint32_t domain_size, const uint32_t* const* columns,
+                                                   uint32_t n_columns, StwoQM31 random_coeff, uint32_t n_batches,
+                                                   const StwoCirclePointQM31* points, const uint32_t* batch_sizes,
+                                                   const uint32_t* column_indexes, const StwoQM31* values,
+                                                   uint32_t* const* out) {
+  return guarded([&] {
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < n_batches; b++) total += batch_sizes[b];
+    for (uint64_t i = 0; i < total; i++)
+      require(column_indexes[i] < n_columns, "accumulate_quotients: column index out of range");
+    DevBuf table = upload(columns, (size_t)n_columns * sizeof(void*));
+    uint32_t* o[4] = {out[0], out[1], out[2], out[3]};
+    accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
+                               table.as<const uint32_t*>(), qm31{{random_coeff.a, random_coeff.b},
+                                                                 {random_coeff.c, random_coeff.d}},
+                               n_batches, points, batch_sizes, column_indexes, values, o);
+  });
+}
+
 extern "C" void accumulate_quotients(uint32_t half_coset_initial_index, uint32_t half_coset_step_size,
                                      uint32_t domain_size, uint32_t* const* columns, size_t number_of_columns,
                                      StwoQM31 random_coeff, const uint32_t* sample_points,

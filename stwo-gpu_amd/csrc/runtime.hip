@@ -2,6 +2,7 @@
 // entry points of the C ABI (cuda/src/utils.cu:10-80 in the reference).
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -150,9 +151,28 @@ void Pool::release_cached() {
   cached_ = 0;
 }
 
+static Staging& take_staging(size_t bytes) {
+  Context& c = g_ctx;
+  for (auto& st : c.staging) {
+    if (st.busy && hipEventQuery(st.done) == hipSuccess) st.busy = false;
+    if (!st.busy && st.cap >= bytes) return st;
+  }
+  Staging st{};
+  st.cap = std::max<size_t>(bytes, 64 << 10);
+  hip_check(hipHostMalloc(&st.host, st.cap, hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipEventCreateWithFlags(&st.done, hipEventDisableTiming), "hipEventCreate");
+  c.staging.push_back(st);
+  return c.staging.back();
+}
+
 DevBuf upload(const void* host, size_t bytes) {
   DevBuf b(bytes);
-  hip_check(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, g_ctx.stream), "hipMemcpyAsync H2D");
+  if (!bytes) return b;
+  Staging& st = take_staging(bytes);
+  memcpy(st.host, host, bytes);
+  hip_check(hipMemcpyAsync(b.p, st.host, bytes, hipMemcpyHostToDevice, g_ctx.stream), "hipMemcpyAsync H2D");
+  hip_check(hipEventRecord(st.done, g_ctx.stream), "hipEventRecord");
+  st.busy = true;
   return b;
 }
 

@@ -58,6 +58,15 @@ struct TwiddleEntry {
   uint32_t* itw;
 };
 
+// Pinned host staging buffers for host-to-device uploads of small tables; a buffer is
+// reused only after the event recorded behind its copy has completed.
+struct Staging {
+  void* host;
+  size_t cap;
+  hipEvent_t done;
+  bool busy;
+};
+
 struct TimedLaunch {
   const char* name;
   hipEvent_t start, stop;
@@ -72,6 +81,7 @@ struct Context {
   std::vector<TimedLaunch> pending;
   std::map<std::string, std::pair<uint64_t, double>> stats;
   std::vector<hipEvent_t> event_pool;
+  std::vector<Staging> staging;
   hipEvent_t timer_start = nullptr, timer_stop = nullptr;
   hipDeviceProp_t props;
 };

@@ -109,6 +109,10 @@ def _declare(L):
         "stwo_hip_interpolate_columns_oop_async": (C.c_int, [pp, pp, u32p, u32, vp, u64]),
         "stwo_hip_lde_columns_async": (C.c_int, [pp, pp, pp, u32p, u32, u32, vp, vp, u64]),
         "stwo_hip_merkle_commit_async": (C.c_int, [u32, pp, u32p, pp, u32]),
+        "stwo_hip_fold_line_async": (C.c_int, [vp, sz, sz, pp, QM31, pp]),
+        "stwo_hip_fold_circle_into_line_async": (C.c_int, [vp, sz, sz, pp, QM31, pp]),
+        "stwo_hip_accumulate_quotients_async": (C.c_int, [u32, u32, u32, pp, u32, QM31, u32, vp, u32p, u32p,
+                                                          C.POINTER(QM31), pp]),
         "stwo_hip_merkle_root": (C.c_int, [vp, vp]),
         "stwo_hip_timer_begin": (C.c_int, []),
         "stwo_hip_timer_end": (C.c_int, [C.POINTER(C.c_float)]),
@@ -545,6 +549,81 @@ class MerkleTree:
         out = np.zeros(8, np.uint32)
         _c(lib().stwo_hip_merkle_root(self.layers[0].ptr, out.ctypes.data))
         return out
+
+
+def fold_line_async(src: SecureColumn, dst: SecureColumn, alpha, itw_ptr, twiddle_offset):
+    _c(lib().stwo_hip_fold_line_async(itw_ptr, twiddle_offset, len(src), src.ptrs(), qm31(alpha), dst.ptrs()))
+
+
+def fold_circle_into_line_async(dst: SecureColumn, src: SecureColumn, alpha, itw_ptr, twiddle_offset):
+    _c(lib().stwo_hip_fold_circle_into_line_async(itw_ptr, twiddle_offset, len(src), src.ptrs(), qm31(alpha),
+                                                  dst.ptrs()))
+
+
+class QuotientSamples:
+    """Host-side flattening of ColumnSampleBatch lists (quotient.rs:55-88) for the async call."""
+
+    def __init__(self, sample_batches):
+        nb = len(sample_batches)
+        self.n = nb
+        self.points = (PointQM31 * max(1, nb))()
+        for i, ((px, py), _) in enumerate(sample_batches):
+            self.points[i] = PointQM31(qm31(px), qm31(py))
+        self.sizes = np.array([len(cv) for _, cv in sample_batches] or [0], np.uint32)
+        self.idx = np.array([c for _, cv in sample_batches for c, _ in cv] or [0], np.uint32)
+        self.vals = (QM31 * max(1, len(self.idx)))(*[qm31(v) for _, cv in sample_batches for _, v in cv])
+
+
+def accumulate_quotients_async(domain_half_coset: Coset, columns, random_coeff, samples: QuotientSamples,
+                               out: SecureColumn):
+    n = 2 * domain_half_coset.size()
+    _c(lib().stwo_hip_accumulate_quotients_async(
+        domain_half_coset.initial_index, domain_half_coset.step_size, n, _ptr_array([c.ptr for c in columns]),
+        len(columns), qm31(random_coeff), samples.n, C.cast(samples.points, vp), samples.sizes.ctypes.data_as(u32p),
+        samples.idx.ctypes.data_as(u32p), samples.vals, out.ptrs()))
+
+
+# ----------------------------------------------------------------- host QM31 helpers
+def qm_mul(p, q):
+    """QM31 product of 4-tuples (a + b i) + (c + d i) u, u^2 = 2 + i."""
+    def cm(x, y):
+        return ((x[0] * y[0] - x[1] * y[1]) % P, (x[0] * y[1] + x[1] * y[0]) % P)
+    x1, y1, x2, y2 = (p[0], p[1]), (p[2], p[3]), (q[0], q[1]), (q[2], q[3])
+    xx, yy = cm(x1, x2), cm(cm(y1, y2), (2, 1))
+    a, b = cm(x1, y2), cm(y1, x2)
+    return ((xx[0] + yy[0]) % P, (xx[1] + yy[1]) % P, (a[0] + b[0]) % P, (a[1] + b[1]) % P)
+
+
+def qm_pow(p, e):
+    r = (1, 0, 0, 0)
+    while e:
+        if e & 1:
+            r = qm_mul(r, p)
+        p = qm_mul(p, p)
+        e >>= 1
+    return r
+
+
+def qm_inv(p):
+    return qm_pow(p, P ** 4 - 2)
+
+
+def random_qm31(rng):
+    return tuple(int(rng.integers(0, P)) for _ in range(4))
+
+
+def random_circle_point_qm31(rng):
+    """Stereographic point: t -> ((1 - t^2)/(1 + t^2), 2t/(1 + t^2))."""
+    while True:
+        t = random_qm31(rng)
+        t2 = qm_mul(t, t)
+        den = ((1 + t2[0]) % P, t2[1], t2[2], t2[3])
+        if den == (0, 0, 0, 0):
+            continue
+        inv = qm_inv(den)
+        x = qm_mul(((1 - t2[0]) % P, (-t2[1]) % P, (-t2[2]) % P, (-t2[3]) % P), inv)
+        y = qm_mul(tuple(2 * v % P for v in t), inv)
+        return x, y
 
 
 def timer_begin():

@@ -423,3 +423,43 @@ def test_lde_columns_fused(log_n, blow, n_cols, in_place):
         assert (ldes[c].to_numpy() == ol[c]).all()
         if not in_place:
             assert (src[c].to_numpy() == evals[c]).all()  # inputs untouched
+
+
+def test_async_fri_chain_matches_oracle():
+    log_n = 12
+    src = _rand_secure(1 << log_n, 4242)
+    hc = S.canonic_half_coset(log_n)
+    tw, itw = S.cached_twiddles(hc)
+    tree_len = 1 << hc.log_size
+    rng = np.random.default_rng(1)
+    alphas = [S.random_qm31(rng) for _ in range(4)]
+    layers = [S.SecureColumn.zeros(1 << l) for l in range(log_n - 1, log_n - 5, -1)]
+    S.fold_circle_into_line_async(layers[0], S.SecureColumn.from_numpy(src), alphas[0], itw, tree_len - len(layers[0]))
+    for i in range(1, len(layers)):
+        S.fold_line_async(layers[i - 1], layers[i], alphas[i], itw, tree_len - len(layers[i - 1]))
+    S.synchronize()
+    o = O.fold_circle_into_line(np.zeros((4, 1 << (log_n - 1)), np.uint32), src, hc.initial_index, hc.step_size,
+                                alphas[0])
+    assert (layers[0].to_numpy() == o).all()
+    d = S.Coset(hc.initial_index, hc.step_size, hc.log_size)
+    for i in range(1, len(layers)):
+        o = O.fold_line(o, d.initial_index, d.step_size, alphas[i])
+        d = d.double()
+        assert (layers[i].to_numpy() == o).all()
+
+
+def test_async_quotients_match_oracle():
+    rng = np.random.default_rng(11)
+    m, n_cols = 13, 12
+    root = S.canonic_half_coset(m)
+    cols = [rand_m31(1 << m, 900 + i) for i in range(n_cols)]
+    pts = [S.random_circle_point_qm31(rng) for _ in range(3)]
+    batches = [(pts[b], [(int(c), S.random_qm31(rng)) for c in sorted(rng.choice(n_cols, size=k, replace=False))])
+               for b, k in enumerate([4, 7, 1])]
+    rc = S.random_qm31(rng)
+    out = S.SecureColumn.uninitialized(1 << m)
+    S.accumulate_quotients_async(root, [S.BaseFieldVec.from_numpy(c) for c in cols], rc, S.QuotientSamples(batches),
+                                 out)
+    S.synchronize()
+    want = O.accumulate_quotients(root.initial_index, root.step_size, m, cols, rc, batches)
+    assert (out.to_numpy() == want).all()
