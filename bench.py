@@ -34,7 +34,9 @@ def algorithmic_bytes(C, n):
     m = 2 * n  # LDE length
     return {
         "fft_inv_contig": 8 * C * n,           # read evaluations, write partial IFFT
+        "fft_inv_strided": 8 * C * n,          # read partial IFFT, write coefficients
         "fft_lde_strided": 4 * C * n + 4 * C * n + 4 * C * m,  # read partial IFFT, write coefficients + partial LDE
+        "fft_fwd_strided": 4 * C * n + 4 * C * m,  # read coefficients (upper half implicit zeros), write partial LDE
         "fft_fwd_contig": 8 * C * m,           # read, write LDE
         "merkle_leaf": 4 * C * m + 32 * m,     # read every LDE value, write 2^21 leaf hashes
     }
@@ -108,7 +110,8 @@ def bench_lde_commit(args):
         step()
     S.synchronize()
     S.kernel_timing(False)
-    names = ["fft_inv_contig", "fft_lde_strided", "fft_fwd_contig", "merkle_leaf", "merkle_subtree"]
+    names = ["fft_inv_contig", "fft_inv_strided", "fft_lde_strided", "fft_fwd_strided", "fft_fwd_contig", "merkle_leaf",
+             "merkle_layer", "merkle_subtree"]
     stats = {k: S.kernel_stats(k) for k in names}
     alg = algorithmic_bytes(C, n)
     kernels = {}
@@ -329,8 +332,121 @@ def bench_quotients(args):
                  _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients"), {"kernels": ks})
 
 
+def cfg4_log_sizes(n_cols=1200, seed=0x5EED0004):
+    """log sizes drawn from {10..22} with weights proportional to 1/size."""
+    import numpy as np
+    ks = np.arange(10, 23)
+    p = 2.0 ** -ks
+    return [int(k) for k in np.random.default_rng(seed).choice(ks, n_cols, p=p / p.sum())]
+
+
+def bench_batched(args):
+    """Mixed-size commit: 1200 columns of log 10..22, LDE at blowup 2 and one Merkle tree with
+    mixed-degree injection.  Batched = one plan (size classes side by side on auxiliary
+    streams, whole sequence replayed as a hipGraph, autotuned pass schedules).  Plain = the
+    per-column trait path through the reference ABI: copy, interpolate, extend (zeroed
+    buffer + copy), evaluate per column, then commit_on_layer per tree layer."""
+    import numpy as np
+    import stwo_hip as S
+
+    logs = cfg4_log_sizes()
+    C = len(logs)
+    L = S.lib()
+    evals = []
+    for c, l in enumerate(logs):
+        v = S.BaseFieldVec.uninitialized(1 << l)
+        S.fill_uniform(v, 0x5EED0004, c << 22)
+        evals.append(v)
+    coeffs = [S.BaseFieldVec.uninitialized(1 << l) for l in logs]
+    ldes = [S.BaseFieldVec.uninitialized(2 << l) for l in logs]
+    max_log = max(logs) + 1
+    root = S.canonic_half_coset(max_log)
+    tw, itw = S.cached_twiddles(root)
+    tree_len = 1 << root.log_size
+    elems = sum(2 << l for l in logs)
+    t0 = time.perf_counter()
+    plan = S.CommitPlan(evals, coeffs, ldes, logs, 1, tw, itw, tree_len, autotune=True)
+    plan_s = time.perf_counter() - t0
+
+    ms_graph = _timed_steps(S, lambda: plan.run(True), args.steps, args.warmup)
+    ms_stream = _timed_steps(S, lambda: plan.run(False), args.steps, args.warmup)
+    root_batched = plan.root()
+    layer0 = plan.layers[0].to_numpy().reshape(-1)
+
+    # plain per-column path (synchronous reference ABI calls, as the trait implementation issues them)
+    p_coeffs = [S.BaseFieldVec.uninitialized(1 << l) for l in logs]
+    by_layer = {}
+    for c, l in enumerate(logs):
+        by_layer.setdefault(l + 1, []).append(c)
+
+    def plain_step():
+        exts = []
+        for c, l in enumerate(logs):
+            n = 1 << l
+            L.copy_uint32_t_vec_from_device_to_device(evals[c].ptr, p_coeffs[c].ptr, n)
+            L.interpolate(n // 2, p_coeffs[c].ptr, itw, tree_len, n)
+            ext = L.cuda_alloc_zeroes_uint32_t(2 * n)
+            L.copy_uint32_t_vec_from_device_to_device(p_coeffs[c].ptr, ext, n)
+            L.evaluate(n, ext, tw, tree_len, 2 * n)
+            exts.append(ext)
+        prev = None
+        for k in range(max_log, -1, -1):
+            cols = [S.BaseFieldVec.view(exts[c], 2 << logs[c]) for c in by_layer.get(k, [])]
+            nxt = S.commit_on_layer(k, prev, cols)
+            prev = nxt
+        for e in exts:
+            L.cuda_free_memory(e)
+        S.check()
+        return prev
+    plain_steps = max(1, min(args.steps, 3))
+    for _ in range(1):
+        plain_step()
+    S.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        top = plain_step()
+    S.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / plain_steps
+    plain_root = top.to_numpy().reshape(-1)
+    pipe = sum(28 * (1 << l) for l in logs) + sum(96 * (1 << k) for k in range(max_log + 1))
+    gbs = pipe / (ms_graph * 1e-3) / 1e9
+    extra = {
+        "batched_graph_ms": round(ms_graph, 4), "batched_stream_ms": round(ms_stream, 4),
+        "plain_per_column_ms": round(ms_plain, 3), "speedup_vs_plain": round(ms_plain / ms_graph, 1),
+        "plan_create_and_autotune_s": round(plan_s, 3), "columns": C,
+        "columns_per_log": {str(k): logs.count(k) for k in range(10, 23)},
+        "root": "".join(f"{w:08x}" for w in root_batched),
+        "plain_root_matches_batched": bool((plain_root == layer0).all()),
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+        host = [O.fill_uniform_m31(1 << l, 0x5EED0004, c << 22) for c, l in enumerate(logs)]
+        t0 = time.perf_counter()
+        olde = [None] * C
+        for k in sorted(set(logs)):
+            idx = [c for c in range(C) if logs[c] == k]
+            _, ls = O.lde_columns([host[c] for c in idx], 1, otw, oitw)
+            for c, v in zip(idx, ls):
+                olde[c] = v
+        oroot = O.merkle_commit(olde)[0]
+        dt = time.perf_counter() - t0
+        extra["parity_root_matches_oracle"] = bool((oroot == root_batched).all())
+        extra["cpu_baseline"] = {"value": elems / dt, "unit": "elems/s", "kind": "port", "sample": "whole config",
+                                 "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    plan.close()
+    return _line("LDE+Merkle M31 elems/s (cfg4 batched mixed sizes)", elems / (ms_graph * 1e-3), "elems/s",
+                 ms_graph, args,
+                 {"workload": f"cfg4: {C} M31 columns, log sizes 10..22 (weights 1/size), LDE blowup 2, one "
+                              "Blake2s tree with mixed-degree injection; hipGraph replay"},
+                 {"bound": "hbm", "kernel": "whole step (all kernels)", "achieved": round(gbs, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                  "algorithmic_bytes_per_launch": pipe, "avg_launch_ms": round(ms_graph, 4)}, extra)
+
+
 CONFIGS = {"cfg2": bench_lde_commit, "cfg1": bench_cfg1, "cfg2-binv": bench_batch_inverse, "cfg2-fri": bench_fri,
-           "cfg3": bench_quotients}
+           "cfg3": bench_quotients, "cfg4": bench_batched}
 
 
 def main():

@@ -244,6 +244,22 @@ int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_index, uint3
                                         const uint32_t *batch_sizes, const uint32_t *column_indexes,
                                         const StwoQM31 *values, uint32_t *const *out);
 
+/* Batched commitment plan: the LDE (interpolate into coeffs[i], which may alias evals[i],
+ * then evaluate on the canonic domain of log log_sizes[i] + log_blowup into ldes[i]) of
+ * columns of any mix of sizes, then the Merkle tree over all LDE columns with
+ * mixed-degree injection into layers[0..max_log] (max_log = largest LDE log size).  The
+ * plan keeps device copies of every pointer table; the buffers must outlive it.
+ * run(use_graph = 1) captures the whole sequence as a hipGraph on first use and replays
+ * it; autotune = 1 times the candidate FFT pass schedules of every size class once and
+ * keeps the fastest (process-wide, per log size, blowup and column-count bucket). */
+typedef struct StwoCommitPlan StwoCommitPlan;
+int stwo_hip_commit_plan_create(StwoCommitPlan **plan, uint32_t n_cols, const uint32_t *const *evals,
+                                uint32_t *const *coeffs, uint32_t *const *ldes, const uint32_t *log_sizes,
+                                uint32_t log_blowup, const uint32_t *twiddles, const uint32_t *itwiddles,
+                                uint64_t tree_len, StwoBlake2sHash *const *layers, uint32_t max_log, int autotune);
+int stwo_hip_commit_plan_run(StwoCommitPlan *plan, int use_graph);
+int stwo_hip_commit_plan_destroy(StwoCommitPlan *plan);
+
 /* Timing on the library stream (HIP events). */
 int stwo_hip_timer_begin(void);
 int stwo_hip_timer_end(float *elapsed_ms);

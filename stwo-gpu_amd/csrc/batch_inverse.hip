@@ -80,13 +80,13 @@ static uint32_t grid_for(uint64_t n, int k) {
 void batch_inverse_m31(const uint32_t* src, uint32_t* dst, uint64_t n) {
   if (!n) return;
   timed("batch_inverse_m31", [&] {
-    binv::m31_kernel<<<binv::grid_for(n, binv::K_M31), 256, 0, ctx().stream>>>(src, dst, n);
+    binv::m31_kernel<<<binv::grid_for(n, binv::K_M31), 256, 0, ctx().cur>>>(src, dst, n);
   });
 }
 void batch_inverse_qm31(const uint32_t* src, uint32_t* dst, uint64_t n) {
   if (!n) return;
   timed("batch_inverse_qm31", [&] {
-    binv::qm31_kernel<<<binv::grid_for(n, binv::K_QM31), 256, 0, ctx().stream>>>((const uint4*)src, (uint4*)dst, n);
+    binv::qm31_kernel<<<binv::grid_for(n, binv::K_QM31), 256, 0, ctx().cur>>>((const uint4*)src, (uint4*)dst, n);
   });
 }
 

@@ -15,6 +15,7 @@
 #include <algorithm>
 
 #include "field.hpp"
+#include "merkle_plan.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -85,9 +86,6 @@ __device__ __forceinline__ void absorb_columns(uint32_t (&h)[8], const uint32_t*
   }
 }
 
-struct Hash {
-  uint32_t s[8];
-};
 
 __device__ __forceinline__ void store_hash(Hash* out, uint32_t i, const uint32_t (&h)[8]) {
   uint4* o = reinterpret_cast<uint4*>(out + i);
@@ -118,12 +116,6 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
   store_hash(out, i, h);
 }
 
-// Columns injected into a layer: device array of column pointers.
-struct LayerCols {
-  const uint32_t* const* cols;
-  uint32_t n_cols;
-  uint32_t pad;
-};
 
 // Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b
 // reads 2^depth children hashes of layer top+1 (prev) and produces 2^(depth-1)
@@ -172,9 +164,14 @@ __global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t
 
 static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
-// Whole tree.  cols/logs are host arrays; layers[k] (host array of device buffers) gets 2^k hashes.
-void commit_tree(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
-                 uint32_t max_log) {
+// A layer with many injected columns is a long per-node hash chain (1 + ceil(cols/16)
+// compressions); it gets its own launch with one lane per node, so all of its nodes
+// run in parallel.  Runs of layers with few columns are built by subtree launches.
+constexpr uint32_t HEAVY_CHUNKS = 2;
+
+MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+                       uint32_t max_log_)
+    : max_log(max_log_) {
   require(max_log <= 31, "merkle: log size too large");
   // stable sort by decreasing size (MerkleProver::commit)
   std::vector<std::vector<const uint32_t*>> by_layer(max_log + 1);
@@ -182,7 +179,6 @@ void commit_tree(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* l
     require(logs[i] <= max_log, "merkle: column larger than the tree");
     by_layer[logs[i]].push_back(cols[i]);
   }
-  // device tables: per-layer column pointers and LayerCols descriptors
   std::vector<const uint32_t*> flat;
   std::vector<size_t> off(max_log + 1);
   for (uint32_t k = 0; k <= max_log; k++) {
@@ -190,30 +186,55 @@ void commit_tree(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* l
     flat.insert(flat.end(), by_layer[k].begin(), by_layer[k].end());
   }
   if (flat.empty()) flat.push_back(nullptr);
-  DevBuf dcols = upload(flat.data(), flat.size() * sizeof(void*));
-  std::vector<LayerCols> lcs(max_log + 1);
+  dcols = upload(flat.data(), flat.size() * sizeof(void*));
+  lcs.resize(max_log + 1);
   for (uint32_t k = 0; k <= max_log; k++)
     lcs[k] = {dcols.as<const uint32_t*>() + off[k], (uint32_t)by_layer[k].size(), 0};
-  DevBuf dlcs = upload(lcs.data(), lcs.size() * sizeof(LayerCols));
-  DevBuf douts = upload(layers, (max_log + 1) * sizeof(Hash*));
-  hipStream_t st = ctx().stream;
-
-  // leaf layer
-  timed("merkle_leaf", [&] {
-    layer_kernel<<<blocks_for(1ull << max_log, 256), 256, 0, st>>>(1u << max_log, lcs[max_log].n_cols,
-                                                                      lcs[max_log].cols, nullptr, layers[max_log]);
-  });
-  // node layers in subtree launches of up to SUB_MAX_DEPTH layers
+  dlcs = upload(lcs.data(), lcs.size() * sizeof(LayerCols));
+  outs.assign(layers, layers + max_log + 1);
+  douts = upload(outs.data(), outs.size() * sizeof(Hash*));
+  // launch schedule, top-down
+  steps.push_back({Step::LAYER, max_log, 1});
   int32_t top = (int32_t)max_log - 1;
   while (top >= 0) {
-    uint32_t depth = std::min<uint32_t>(SUB_MAX_DEPTH, (uint32_t)top + 1);
-    uint32_t blocks = 1u << ((uint32_t)top + 1 - depth);
-    timed("merkle_subtree", [&] {
-      subtree_kernel<<<blocks, 256, 0, st>>>(layers[top + 1], (uint32_t)top, depth, douts.as<Hash* const>(),
-                                             dlcs.as<const LayerCols>());
-    });
+    if ((lcs[top].n_cols + 15) / 16 >= HEAVY_CHUNKS && top >= 8) {
+      steps.push_back({Step::LAYER, (uint32_t)top, 1});
+      top--;
+      continue;
+    }
+    uint32_t depth = 1;
+    while (depth < (uint32_t)SUB_MAX_DEPTH && (int32_t)depth <= top &&
+           !((lcs[top - depth].n_cols + 15) / 16 >= HEAVY_CHUNKS && top - (int32_t)depth >= 8))
+      depth++;
+    steps.push_back({Step::SUBTREE, (uint32_t)top, depth});
     top -= (int32_t)depth;
   }
+}
+
+void MerklePlan::enqueue() const {
+  hipStream_t st = ctx().cur;
+  for (const Step& s : steps) {
+    if (s.kind == Step::LAYER) {
+      const uint32_t k = s.top;
+      const Hash* prev = k == max_log ? nullptr : outs[k + 1];
+      timed(k == max_log ? "merkle_leaf" : "merkle_layer", [&] {
+        layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev, outs[k]);
+      });
+    } else {
+      const uint32_t blocks = 1u << (s.top + 1 - s.depth);
+      timed("merkle_subtree", [&] {
+        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, douts.as<Hash* const>(),
+                                               dlcs.as<const LayerCols>());
+      });
+    }
+  }
+}
+
+// Whole tree.  cols/logs are host arrays; layers[k] (host array of device buffers) gets 2^k hashes.
+void commit_tree(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+                 uint32_t max_log) {
+  MerklePlan plan(n_cols, cols, logs, layers, max_log);
+  plan.enqueue();
 }
 
 }  // namespace merkle
@@ -228,7 +249,7 @@ extern "C" void commit_on_first_layer(size_t size, size_t amount_of_columns, uin
     require(size <= 0xffffffffull, "commit_on_first_layer: size too large");
     if (!size) return;
     timed("merkle_leaf", [&] {
-      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().stream>>>(
+      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(
           (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, nullptr, (Hash*)result);
     });
     sync();
@@ -241,7 +262,7 @@ extern "C" void commit_on_layer_with_previous(size_t size, size_t amount_of_colu
     require(size <= 0xffffffffull, "commit_on_layer_with_previous: size too large");
     if (!size) return;
     timed("merkle_node", [&] {
-      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().stream>>>(
+      merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(
           (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, (const Hash*)previous_layer,
           (Hash*)result);
     });
@@ -256,7 +277,7 @@ extern "C" int stwo_hip_merkle_commit_async(uint32_t n_cols, const uint32_t* con
 
 extern "C" int stwo_hip_merkle_root(const StwoBlake2sHash* root_layer, StwoBlake2sHash* root_out) {
   return guarded([&] {
-    hip_check(hipMemcpyAsync(root_out, root_layer, 32, hipMemcpyDefault, ctx().stream), "hipMemcpy");
+    hip_check(hipMemcpyAsync(root_out, root_layer, 32, hipMemcpyDefault, ctx().cur), "hipMemcpy");
     sync();
   });
 }

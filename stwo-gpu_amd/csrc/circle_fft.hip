@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "fft.hpp"
 #include "field.hpp"
 #include "runtime.hpp"
 
@@ -307,8 +308,7 @@ constexpr int BS_LDE = 14;
 // The fused middle pass saves one HBM read of the coefficients, but it holds the
 // coefficients and the forward tile in registers (4 waves/SIMD instead of 8); on gfx950
 // the separate strided passes measured faster (1.63 ms vs 1.73 ms for 256 columns of log
-// 20), so lde_same_size uses them.
-constexpr bool kFusedLde = false;
+// 20), so it is a variant for the autotuner rather than the default.
 constexpr int NT_S = 1024;  // threads per 2^14-element tile: 16 values per thread
 template <int K>
 __global__ __launch_bounds__(NT_S) void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
@@ -418,7 +418,8 @@ struct PassPlan {
   uint32_t a, k, t, hb, multi, B;
 };
 
-static std::vector<PassPlan> plan(uint32_t log_n) {
+// kc_pref: layers in the contiguous pass (12 or 13; 0 = the fewest passes, then 12).
+static std::vector<PassPlan> plan(uint32_t log_n, uint32_t kc_pref = 0) {
   std::vector<PassPlan> ps;
   if (log_n <= 12) {
     ps.push_back({0, log_n, 0, 12 - log_n, 1, 12});
@@ -428,7 +429,7 @@ static std::vector<PassPlan> plan(uint32_t log_n) {
     ps.push_back({0, 13, 0, 0, 0, 13});
     return ps;
   }
-  uint32_t kc = (log_n - 12 > KS_MAX) ? 13 : 12;
+  uint32_t kc = kc_pref ? kc_pref : ((log_n - 12 > KS_MAX) ? 13 : 12);
   ps.push_back({0, kc, 0, 0, 0, kc});
   uint32_t a = kc;
   while (a < log_n) {
@@ -453,9 +454,9 @@ template <bool INV, int B, int NT, int K, int KIND>
 static void launch_k(uint32_t blocks, const PassArgs& A) {
   // forward passes that reach layers at or above act_log (zero-padded coefficients) skip them
   if (!INV && A.act_log < A.a + K)
-    fft_pass_kernel<false, B, NT, K, KIND, true><<<blocks, NT, 0, ctx().stream>>>(A);
+    fft_pass_kernel<false, B, NT, K, KIND, true><<<blocks, NT, 0, ctx().cur>>>(A);
   else
-    fft_pass_kernel<INV, B, NT, K, KIND, false><<<blocks, NT, 0, ctx().stream>>>(A);
+    fft_pass_kernel<INV, B, NT, K, KIND, false><<<blocks, NT, 0, ctx().cur>>>(A);
 }
 
 template <bool INV, int K>
@@ -509,13 +510,13 @@ static void launch_pass(const PassPlan& p, const PassArgs& base) {
 // Inverse FFT of n_cols columns of one log size (device pointer tables): the first pass
 // reads d_src, every pass writes d_cols (d_src == d_cols: in place).
 void interpolate_same_size(uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_n, const uint32_t* itw,
-                           uint64_t tree_len, const uint32_t* const* d_src = nullptr) {
+                           uint64_t tree_len, const uint32_t* const* d_src = nullptr, uint32_t kc = 0) {
   if (!d_src) d_src = (const uint32_t* const*)d_cols;
   require(log_n >= 1 && log_n <= 30, "interpolate: log size out of range");
   require(tree_len >= (1ull << (log_n - 1)), "interpolate: twiddle tree smaller than the domain");
   if (log_n <= 2) {
     timed("fft_tiny_inv", [&] {
-      fft_tiny_kernel<true><<<ceil_div(n_cols, 64), 64, 0, ctx().stream>>>(d_cols, d_src, n_cols, log_n, log_n);
+      fft_tiny_kernel<true><<<ceil_div(n_cols, 64), 64, 0, ctx().cur>>>(d_cols, d_src, n_cols, log_n, log_n);
     });
     return;
   }
@@ -526,7 +527,7 @@ void interpolate_same_size(uint32_t* const* d_cols, uint32_t n_cols, uint32_t lo
   A.log_n = log_n;
   A.src_log = log_n;
   A.act_log = log_n;
-  auto ps = plan(log_n);
+  auto ps = plan(log_n, kc);
   for (size_t i = 0; i < ps.size(); i++) {
     A.src = i == 0 ? d_src : (const uint32_t* const*)d_cols;
     // scale by 2^-log_n in the last pass: 2^-n = 2^(31 - n) since 2^31 = 1
@@ -538,12 +539,12 @@ void interpolate_same_size(uint32_t* const* d_cols, uint32_t n_cols, uint32_t lo
 // Forward FFT: src columns (2^src_log coefficients) evaluated on the canonic domain
 // of log_n >= src_log, written to dst (device pointer tables).
 void evaluate_same_size(uint32_t* const* d_dst, const uint32_t* const* d_src, uint32_t n_cols, uint32_t src_log,
-                        uint32_t log_n, const uint32_t* tw, uint64_t tree_len) {
+                        uint32_t log_n, const uint32_t* tw, uint64_t tree_len, uint32_t kc = 0) {
   require(log_n >= 1 && log_n <= 30 && src_log <= log_n, "evaluate: log sizes out of range");
   require(tree_len >= (1ull << (log_n - 1)), "evaluate: twiddle tree smaller than the domain");
   if (log_n <= 2) {
     timed("fft_tiny_fwd", [&] {
-      fft_tiny_kernel<false><<<ceil_div(n_cols, 64), 64, 0, ctx().stream>>>(d_dst, d_src, n_cols, log_n, src_log);
+      fft_tiny_kernel<false><<<ceil_div(n_cols, 64), 64, 0, ctx().cur>>>(d_dst, d_src, n_cols, log_n, src_log);
     });
     return;
   }
@@ -553,7 +554,7 @@ void evaluate_same_size(uint32_t* const* d_dst, const uint32_t* const* d_src, ui
   A.n_cols = n_cols;
   A.log_n = log_n;
   A.scale = 0;
-  auto ps = plan(log_n);
+  auto ps = plan(log_n, kc);
   for (size_t i = ps.size(); i-- > 0;) {
     bool first = (i + 1 == ps.size());
     A.src = first ? d_src : (const uint32_t* const*)d_dst;
@@ -567,7 +568,7 @@ template <int K>
 static void launch_lde_strided(uint32_t blocks, const PassArgs& Ai, const PassArgs& Af, uint32_t n_blocks, uint32_t k) {
   if constexpr (K >= 2) {
     if (k == K) {
-      fft_lde_strided_kernel<K><<<blocks, NT_S, 0, ctx().stream>>>(Ai, Af, n_blocks);
+      fft_lde_strided_kernel<K><<<blocks, NT_S, 0, ctx().cur>>>(Ai, Af, n_blocks);
       return;
     }
     launch_lde_strided<K - 1>(blocks, Ai, Af, n_blocks, k);
@@ -577,13 +578,16 @@ static void launch_lde_strided(uint32_t blocks, const PassArgs& Ai, const PassAr
 // Interpolate d_src (2^n evaluations) into d_coeffs and evaluate on the canonic domain of
 // log n + log_blowup into d_lde.  d_src may equal d_coeffs.
 void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint32_t* const* d_lde, uint32_t n_cols,
-                   uint32_t n, uint32_t log_blowup, const uint32_t* tw, const uint32_t* itw, uint64_t tree_len) {
+                   uint32_t n, uint32_t log_blowup, const uint32_t* tw, const uint32_t* itw, uint64_t tree_len,
+                   LdeVariant var) {
   const uint32_t m = n + log_blowup;
   require(m <= 30, "lde: log size out of range");
   require(tree_len >= (1ull << (m - 1)), "lde: twiddle tree smaller than the evaluation domain");
-  if (!kFusedLde || n < 14 || n > 12 + KS_MAX) {
-    interpolate_same_size(d_coeffs, n_cols, n, itw, tree_len, d_src);
-    evaluate_same_size(d_lde, (const uint32_t* const*)d_coeffs, n_cols, n, m, tw, tree_len);
+  if (!var.fused || n < 14 || n > 12 + KS_MAX) {
+    const uint32_t kci = (var.kc == 13 && n >= 14 && n - 13 <= KS_MAX) ? 13 : 0;
+    const uint32_t kcf = (var.kc == 13 && m >= 14 && m - 13 <= KS_MAX) ? 13 : 0;
+    interpolate_same_size(d_coeffs, n_cols, n, itw, tree_len, d_src, kci);
+    evaluate_same_size(d_lde, (const uint32_t* const*)d_coeffs, n_cols, n, m, tw, tree_len, kcf);
     return;
   }
   const uint32_t k = n - 12, t = BS_LDE - k;
@@ -649,7 +653,7 @@ void lde_mixed(const uint32_t* const* src, uint32_t* const* coeffs, uint32_t* co
     DevBuf d = upload(t.data(), t.size() * sizeof(void*));
     const void* const* base = d.as<const void*>();
     lde_same_size((const uint32_t* const*)base, (uint32_t* const*)(base + cnt), (uint32_t* const*)(base + 2 * cnt),
-                  (uint32_t)cnt, kv.first, log_blowup, tw, itw, tree_len);
+                  (uint32_t)cnt, kv.first, log_blowup, tw, itw, tree_len, lde_variant_for(kv.first, log_blowup, cnt));
   }
 }
 

@@ -61,11 +61,11 @@ void bit_reverse(T* a, uint64_t n) {
   uint32_t log_n = log2_exact(n);
   if (log_n >= 2 * TA) {
     timed("bit_reverse", [&] {
-      bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), 256, 0, ctx().stream>>>(a, log_n);
+      bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), 256, 0, ctx().cur>>>(a, log_n);
     });
   } else if (n > 1) {
     timed("bit_reverse", [&] {
-      bitrev_simple_kernel<T><<<ceil_div(n, 256), 256, 0, ctx().stream>>>(a, (uint32_t)n, log_n);
+      bitrev_simple_kernel<T><<<ceil_div(n, 256), 256, 0, ctx().cur>>>(a, (uint32_t)n, log_n);
     });
   }
 }
@@ -111,7 +111,7 @@ extern "C" uint32_t* sort_values_and_permute_with_bit_reverse_order(const uint32
     require(is_pow2(size) && size <= (1ull << 31), "sort_values: size must be a power of two");
     dst = (uint32_t*)ctx().pool.alloc(size * 4);
     timed("sort_values", [&] {
-      cols::sort_values_kernel<<<ceil_div(size, 256), 256, 0, ctx().stream>>>(from, dst, (uint32_t)size);
+      cols::sort_values_kernel<<<ceil_div(size, 256), 256, 0, ctx().cur>>>(from, dst, (uint32_t)size);
     });
     cols::bit_reverse(dst, size);
     sync();
@@ -124,7 +124,7 @@ extern "C" void accumulate(uint32_t size, uint32_t* const* left_columns, uint32_
     cols::Soa4 l{{left_columns[0], left_columns[1], left_columns[2], left_columns[3]}};
     cols::Soa4 r{{right_columns[0], right_columns[1], right_columns[2], right_columns[3]}};
     timed("accumulate", [&] {
-      cols::accumulate_kernel<<<ceil_div(size, 256), 256, 0, ctx().stream>>>(size, l, r);
+      cols::accumulate_kernel<<<ceil_div(size, 256), 256, 0, ctx().cur>>>(size, l, r);
     });
     sync();
   });

@@ -1,0 +1,222 @@
+// Batched commitment of columns of many log sizes: one plan object per column set
+// (the Rust trait path builds the same commitment with one synchronous call per
+// operation, per column batch of equal size: stwo_gpu_backend/src/poly.rs:57-168 and
+// blake2s.rs:19-97, driven layer by layer by stwo's MerkleProver::commit).
+//
+// The plan uploads every pointer table once, then each run enqueues:
+//   * the LDE of each size class (interpolate into coefficients, evaluate on the
+//     blowup domain), the classes fanned out over the auxiliary streams so small
+//     classes run side by side, joined back into the library stream;
+//   * the whole Merkle tree over the LDE columns with mixed-degree injection.
+// A run can be captured once as a hipGraph and replayed (no per-launch CPU cost).
+// Nothing is allocated while enqueuing, and no memset is recorded.
+//
+// Autotuner: for each size class of log >= 14 the candidate pass schedules
+// (contiguous passes of 12 or 13 layers, strided passes fused or not) are timed on
+// the plan's own buffers and the fastest is kept, process-wide, for that
+// (log size, blowup, column-count bucket).
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "fft.hpp"
+#include "merkle_plan.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace fft {
+
+namespace {
+std::map<std::tuple<uint32_t, uint32_t, uint32_t>, LdeVariant> g_variants;
+uint32_t bucket(uint64_t n_cols) { return log2_exact(n_cols ? n_cols : 1); }
+}  // namespace
+
+LdeVariant lde_variant_for(uint32_t n, uint32_t log_blowup, uint64_t n_cols) {
+  auto it = g_variants.find({n, log_blowup, bucket(n_cols)});
+  return it == g_variants.end() ? LdeVariant{} : it->second;
+}
+void set_lde_variant(uint32_t n, uint32_t log_blowup, uint64_t n_cols, LdeVariant v) {
+  g_variants[{n, log_blowup, bucket(n_cols)}] = v;
+}
+
+}  // namespace fft
+
+struct CommitPlan {
+  struct Class {
+    uint32_t log, n;
+    DevBuf tables;  // [src | coeffs | lde] device pointer tables
+    fft::LdeVariant var;
+  };
+  std::vector<Class> classes;
+  uint32_t log_blowup;
+  const uint32_t *tw, *itw;
+  uint64_t tree_len;
+  std::unique_ptr<merkle::MerklePlan> merkle;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void enqueue_class(const Class& c) const {
+    const void* const* base = c.tables.as<const void*>();
+    fft::lde_same_size((const uint32_t* const*)base, (uint32_t* const*)(base + c.n),
+                       (uint32_t* const*)(base + 2 * c.n), c.n, c.log, log_blowup, tw, itw, tree_len, c.var);
+  }
+
+  void enqueue() const {
+    Context& x = ctx();
+    const int n_aux = (int)std::min<size_t>(classes.size(), N_AUX_STREAMS);
+    if (n_aux > 1) {
+      struct Restore {
+        Context& x;
+        ~Restore() { x.cur = x.stream; }
+      } restore{x};
+      hip_check(hipEventRecord(x.fork, x.stream), "hipEventRecord");
+      for (int i = 0; i < n_aux; i++) hip_check(hipStreamWaitEvent(x.aux[i], x.fork, 0), "hipStreamWaitEvent");
+      // largest classes first, round robin over the auxiliary streams
+      for (size_t i = 0; i < classes.size(); i++) {
+        x.cur = x.aux[i % n_aux];
+        enqueue_class(classes[i]);
+      }
+      x.cur = x.stream;
+      for (int i = 0; i < n_aux; i++) {
+        hip_check(hipEventRecord(x.join[i], x.aux[i]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(x.stream, x.join[i], 0), "hipStreamWaitEvent");
+      }
+    } else {
+      for (const Class& c : classes) enqueue_class(c);
+    }
+    merkle->enqueue();
+  }
+
+  void run(bool use_graph) {
+    Context& x = ctx();
+    if (!use_graph) {
+      enqueue();
+      return;
+    }
+    if (!exec) {
+      const bool timing = x.kernel_timing;
+      x.kernel_timing = false;  // no event bookkeeping inside a captured graph
+      hip_check(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        enqueue();
+      } catch (...) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(x.stream, &g);
+        x.cur = x.stream;
+        x.kernel_timing = timing;
+        throw;
+      }
+      hip_check(hipStreamEndCapture(x.stream, &graph), "hipStreamEndCapture");
+      x.kernel_timing = timing;
+      hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(exec, x.stream), "hipGraphLaunch");
+  }
+
+  // time one class's LDE under each candidate schedule; keep the fastest
+  void autotune() {
+    Context& x = ctx();
+    for (Class& c : classes) {
+      if (c.log < 14) continue;
+      std::vector<fft::LdeVariant> cand = {{12, false}};
+      if (c.log - 13 <= 9 && c.log + log_blowup - 13 <= 9) cand.push_back({13, false});
+      if (c.log <= 21) cand.push_back({12, true});
+      if (cand.size() == 1) continue;
+      float best = 1e30f;
+      fft::LdeVariant best_v = cand[0];
+      for (const auto& v : cand) {
+        c.var = v;
+        enqueue_class(c);  // warm-up
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+          hip_check(hipEventRecord(x.timer_start, x.stream), "hipEventRecord");
+          enqueue_class(c);
+          hip_check(hipEventRecord(x.timer_stop, x.stream), "hipEventRecord");
+          hip_check(hipEventSynchronize(x.timer_stop), "hipEventSynchronize");
+          float ms = 0;
+          hip_check(hipEventElapsedTime(&ms, x.timer_start, x.timer_stop), "hipEventElapsedTime");
+          ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best) {
+          best = ms_min;
+          best_v = v;
+        }
+      }
+      c.var = best_v;
+      fft::set_lde_variant(c.log, log_blowup, c.n, best_v);
+    }
+  }
+
+  ~CommitPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+}  // namespace sh
+
+using namespace sh;
+
+struct StwoCommitPlan {
+  CommitPlan plan;
+};
+
+extern "C" int stwo_hip_commit_plan_create(StwoCommitPlan** out, uint32_t n_cols, const uint32_t* const* evals,
+                                           uint32_t* const* coeffs, uint32_t* const* ldes, const uint32_t* log_sizes,
+                                           uint32_t log_blowup, const uint32_t* twiddles, const uint32_t* itwiddles,
+                                           uint64_t tree_len, StwoBlake2sHash* const* layers, uint32_t max_log,
+                                           int autotune) {
+  return guarded([&] {
+    require(out != nullptr && n_cols > 0, "commit plan: no columns");
+    auto holder = std::make_unique<StwoCommitPlan>();
+    CommitPlan& p = holder->plan;
+    p.log_blowup = log_blowup;
+    p.tw = twiddles;
+    p.itw = itwiddles;
+    p.tree_len = tree_len;
+    std::map<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> by;
+    uint32_t top = 0;
+    for (uint32_t i = 0; i < n_cols; i++) {
+      require(log_sizes[i] >= 1 && log_sizes[i] + log_blowup <= 30, "commit plan: log size out of range");
+      require(tree_len >= (1ull << (log_sizes[i] + log_blowup - 1)), "commit plan: twiddle tree too small");
+      by[log_sizes[i]].push_back(i);
+      top = std::max(top, log_sizes[i] + log_blowup);
+    }
+    require(max_log == top, "commit plan: max_log must be the largest LDE log size");
+    for (auto& kv : by) {
+      const uint32_t n = (uint32_t)kv.second.size();
+      std::vector<const void*> t(3 * n);
+      for (uint32_t i = 0; i < n; i++) {
+        t[i] = evals[kv.second[i]];
+        t[n + i] = coeffs[kv.second[i]];
+        t[2 * n + i] = ldes[kv.second[i]];
+      }
+      CommitPlan::Class c{kv.first, n, upload(t.data(), t.size() * sizeof(void*)),
+                          fft::lde_variant_for(kv.first, log_blowup, n)};
+      p.classes.push_back(std::move(c));
+    }
+    std::vector<uint32_t> lde_logs(n_cols);
+    for (uint32_t i = 0; i < n_cols; i++) lde_logs[i] = log_sizes[i] + log_blowup;
+    p.merkle = std::make_unique<merkle::MerklePlan>(n_cols, (const uint32_t* const*)ldes, lde_logs.data(),
+                                                    (merkle::Hash* const*)layers, max_log);
+    if (autotune) p.autotune();
+    sync();
+    *out = holder.release();
+  });
+}
+
+extern "C" int stwo_hip_commit_plan_run(StwoCommitPlan* plan, int use_graph) {
+  return guarded([&] {
+    require(plan != nullptr, "commit plan: null plan");
+    plan->plan.run(use_graph != 0);
+  });
+}
+
+extern "C" int stwo_hip_commit_plan_destroy(StwoCommitPlan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    sync();
+    delete plan;
+  });
+}

@@ -92,7 +92,7 @@ qm31 eval_at_point_sync(const uint32_t* coeffs, uint64_t n, qm31 px, qm31 py) {
   uint32_t blocks = 1u << (L - cb);
   DevBuf pa(sizeof(qm31) * blocks), pb(sizeof(qm31) * blocks);
   timed("eval_at_point", [&] {
-    eap::first_pass<<<blocks, eap::NT, 0, ctx().stream>>>(coeffs, L, cb, F, pa.as<qm31>());
+    eap::first_pass<<<blocks, eap::NT, 0, ctx().cur>>>(coeffs, L, cb, F, pa.as<qm31>());
   });
   int fi = (int)(L - cb) - 1;
   while (blocks > 1) {
@@ -100,7 +100,7 @@ qm31 eval_at_point_sync(const uint32_t* coeffs, uint64_t n, qm31 px, qm31 py) {
     uint32_t g = std::min<uint32_t>(lb, eap::CB - 1);
     uint32_t nb = blocks >> g;
     timed("eval_at_point", [&] {
-      eap::reduce_pass<<<nb, eap::NT, 0, ctx().stream>>>(pa.as<qm31>(), g, F, fi, pb.as<qm31>());
+      eap::reduce_pass<<<nb, eap::NT, 0, ctx().cur>>>(pa.as<qm31>(), g, F, fi, pb.as<qm31>());
     });
     std::swap(pa.p, pb.p);
     fi -= (int)g;

@@ -71,7 +71,7 @@ void fold_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint3
   require(n >= 2 && is_pow2(n) && n <= (1ull << 31), "fold_line: n must be a power of two >= 2");
   uint32_t half = (uint32_t)(n >> 1);
   timed("fold_line", [&] {
-    fri::fold_line_kernel<<<ceil_div(half, 256), 256, 0, ctx().stream>>>(itw + twiddle_offset, half, fri::soa(src),
+    fri::fold_line_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(itw + twiddle_offset, half, fri::soa(src),
                                                                            alpha, fri::soa(dst));
   });
 }
@@ -81,7 +81,7 @@ void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, siz
   uint32_t half = (uint32_t)(n >> 1);
   qm31 a2 = qmul(alpha, alpha);
   timed("fold_circle_into_line", [&] {
-    fri::fold_circle_kernel<<<ceil_div(half, 256), 256, 0, ctx().stream>>>(itw + twiddle_offset, half,
+    fri::fold_circle_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(itw + twiddle_offset, half,
                                                                              fri::soa(src), alpha, a2, fri::soa(dst));
   });
 }

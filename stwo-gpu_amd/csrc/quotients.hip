@@ -132,7 +132,7 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
   DevBuf dss = upload(ss.data(), ss.size() * sizeof(quot::Sample));
   DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
   timed("accumulate_quotients", [&] {
-    quot::accumulate_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().stream>>>(
+    quot::accumulate_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().cur>>>(
         hi, hs, log_size, d_cols, dss.as<quot::Sample>(), dbs.as<quot::Batch>(), n_batches, out[0], out[1], out[2],
         out[3]);
   });

@@ -50,6 +50,12 @@ static void init_context(int device) {
     fail(STWO_HIP_ERR_NODEV, "device %d is %s; this library is built for gfx950", device, c.props.gcnArchName);
   // default priority, non-blocking w.r.t. the legacy stream
   hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+  c.cur = c.stream;
+  for (int i = 0; i < N_AUX_STREAMS; i++) {
+    hip_check(hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&c.join[i], hipEventDisableTiming), "hipEventCreate");
+  }
+  hip_check(hipEventCreateWithFlags(&c.fork, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipEventCreate(&c.timer_start), "hipEventCreate");
   hip_check(hipEventCreate(&c.timer_stop), "hipEventCreate");
   c.device = device;
@@ -97,13 +103,13 @@ void launch_begin(const char* name) {
   Context& c = g_ctx;
   if (!c.kernel_timing) return;
   TimedLaunch t{name, take_event(), take_event()};
-  hip_check(hipEventRecord(t.start, c.stream), "hipEventRecord");
+  hip_check(hipEventRecord(t.start, c.cur), "hipEventRecord");
   c.pending.push_back(t);
 }
 void launch_end(const char* name) {
   Context& c = g_ctx;
   if (!c.kernel_timing || c.pending.empty()) return;
-  hip_check(hipEventRecord(c.pending.back().stop, c.stream), "hipEventRecord");
+  hip_check(hipEventRecord(c.pending.back().stop, c.cur), "hipEventRecord");
 }
 
 // ---- pool ----
@@ -170,8 +176,8 @@ DevBuf upload(const void* host, size_t bytes) {
   if (!bytes) return b;
   Staging& st = take_staging(bytes);
   memcpy(st.host, host, bytes);
-  hip_check(hipMemcpyAsync(b.p, st.host, bytes, hipMemcpyHostToDevice, g_ctx.stream), "hipMemcpyAsync H2D");
-  hip_check(hipEventRecord(st.done, g_ctx.stream), "hipEventRecord");
+  hip_check(hipMemcpyAsync(b.p, st.host, bytes, hipMemcpyHostToDevice, g_ctx.cur), "hipMemcpyAsync H2D");
+  hip_check(hipEventRecord(st.done, g_ctx.cur), "hipEventRecord");
   st.busy = true;
   return b;
 }
@@ -249,7 +255,7 @@ extern "C" int stwo_hip_fill_uniform_m31(uint32_t* dst, uint64_t n, uint64_t see
     if (!n) return;
     uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 256u * 64u);
     timed("fill_uniform", [&] {
-      fill_uniform_kernel<<<blocks, 256, 0, ctx().stream>>>(dst, n, seed, offset);
+      fill_uniform_kernel<<<blocks, 256, 0, ctx().cur>>>(dst, n, seed, offset);
     });
   });
 }
@@ -260,7 +266,7 @@ extern "C" int stwo_hip_device_copy(void* dst, const void* src, size_t bytes) {
     if (!n16) return;
     uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 256u * 32u);
     timed("copy16", [&] {
-      copy16_kernel<<<blocks, 256, 0, ctx().stream>>>((const uint4*)src, (uint4*)dst, n16);
+      copy16_kernel<<<blocks, 256, 0, ctx().cur>>>((const uint4*)src, (uint4*)dst, n16);
     });
   });
 }

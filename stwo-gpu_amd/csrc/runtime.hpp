@@ -72,9 +72,15 @@ struct TimedLaunch {
   hipEvent_t start, stop;
 };
 
+constexpr int N_AUX_STREAMS = 4;
+
 struct Context {
   int device = -1;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // the library stream: every call is ordered on it
+  hipStream_t cur = nullptr;     // where kernels are launched now (the library stream, or an
+                                 // auxiliary stream while a plan fans independent work out)
+  hipStream_t aux[N_AUX_STREAMS] = {};
+  hipEvent_t fork = nullptr, join[N_AUX_STREAMS] = {};
   Pool pool;
   std::map<TwiddleKey, TwiddleEntry> twiddles;
   bool kernel_timing = false;
@@ -99,6 +105,14 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   DevBuf(DevBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) ctx().pool.free(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) ctx().pool.free(p);
   }

@@ -49,7 +49,7 @@ __global__ void tree_kernel(uint32_t* out, pt initial, pt step, uint32_t log_siz
 void compute_tree(uint32_t* out, pt initial, pt step, uint32_t log_size) {
   uint32_t size = 1u << log_size;
   timed("twiddle_tree", [&] {
-    tree_kernel<<<ceil_div(size, 256), 256, 0, ctx().stream>>>(out, initial, step, log_size);
+    tree_kernel<<<ceil_div(size, 256), 256, 0, ctx().cur>>>(out, initial, step, log_size);
   });
 }
 

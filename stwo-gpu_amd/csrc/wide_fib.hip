@@ -57,7 +57,7 @@ void wide_fib_quotients_async(uint32_t* const q[4], const uint32_t* const* d_tra
                               const uint32_t* dinv, uint32_t n_rows, uint32_t n_cols) {
   require(n_cols >= 3, "wide fibonacci: needs at least 3 columns");
   timed("wide_fib_quotients", [&] {
-    wfib::quotients_kernel<<<ceil_div(n_rows, 256), 256, 0, ctx().stream>>>(
+    wfib::quotients_kernel<<<ceil_div(n_rows, 256), 256, 0, ctx().cur>>>(
         n_rows, n_cols, d_trace, (const uint4*)powers, dinv, q[0], q[1], q[2], q[3]);
   });
 }

@@ -114,6 +114,10 @@ def _declare(L):
         "stwo_hip_accumulate_quotients_async": (C.c_int, [u32, u32, u32, pp, u32, QM31, u32, vp, u32p, u32p,
                                                           C.POINTER(QM31), pp]),
         "stwo_hip_merkle_root": (C.c_int, [vp, vp]),
+        "stwo_hip_commit_plan_create": (C.c_int, [C.POINTER(vp), u32, pp, pp, pp, u32p, u32, vp, vp, u64, pp, u32,
+                                                  C.c_int]),
+        "stwo_hip_commit_plan_run": (C.c_int, [vp, C.c_int]),
+        "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
         "stwo_hip_timer_begin": (C.c_int, []),
         "stwo_hip_timer_end": (C.c_int, [C.POINTER(C.c_float)]),
         "stwo_hip_kernel_timing": (C.c_int, [C.c_int]),
@@ -174,9 +178,9 @@ class DeviceBuffer:
         self.nbytes = nbytes
 
     def free(self):
-        if self.ptr:
+        if self.ptr and not getattr(self, "_borrowed", False):
             lib().cuda_free_memory(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -204,6 +208,13 @@ class BaseFieldVec(DeviceBuffer):
         ptr = lib().cuda_malloc_uint32_t(size)
         check()
         return cls(ptr, size)
+
+    @classmethod
+    def view(cls, ptr, size):
+        """Non-owning view of device memory owned elsewhere (never freed through this object)."""
+        v = cls(ptr, size)
+        v._borrowed = True
+        return v
 
     @classmethod
     def zeros(cls, size):
@@ -624,6 +635,42 @@ def random_circle_point_qm31(rng):
         x = qm_mul(((1 - t2[0]) % P, (-t2[1]) % P, (-t2[2]) % P, (-t2[3]) % P), inv)
         y = qm_mul(tuple(2 * v % P for v in t), inv)
         return x, y
+
+
+class CommitPlan:
+    """Batched LDE + Merkle commitment of columns of mixed sizes (stwo_hip_commit_plan_*)."""
+
+    def __init__(self, evals, coeffs, ldes, log_sizes, log_blowup, tw_ptr, itw_ptr, tree_len, autotune=False):
+        self.log_sizes = np.ascontiguousarray(log_sizes, np.uint32)
+        self.max_log = int(self.log_sizes.max()) + log_blowup
+        self.layers = [Blake2sHashVec.uninitialized(1 << k) for k in range(self.max_log + 1)]
+        self._keep = (evals, coeffs, ldes)
+        h = vp()
+        _c(lib().stwo_hip_commit_plan_create(C.byref(h), len(evals), _ptr_array([c.ptr for c in evals]),
+                                             _ptr_array([c.ptr for c in coeffs]), _ptr_array([c.ptr for c in ldes]),
+                                             self.log_sizes.ctypes.data_as(u32p), log_blowup, tw_ptr, itw_ptr,
+                                             tree_len, _ptr_array([l.ptr for l in self.layers]), self.max_log,
+                                             1 if autotune else 0))
+        self.handle = h.value
+
+    def run(self, use_graph=True):
+        _c(lib().stwo_hip_commit_plan_run(self.handle, 1 if use_graph else 0))
+
+    def root(self):
+        out = np.zeros(8, np.uint32)
+        _c(lib().stwo_hip_merkle_root(self.layers[0].ptr, out.ctypes.data))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().stwo_hip_commit_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def timer_begin():

@@ -463,3 +463,38 @@ def test_async_quotients_match_oracle():
     S.synchronize()
     want = O.accumulate_quotients(root.initial_index, root.step_size, m, cols, rc, batches)
     assert (out.to_numpy() == want).all()
+
+
+# ------------------------------------------------------------------ batched commit (poly.rs:57-168 + blake2s.rs:19-97)
+@pytest.mark.parametrize("use_graph,autotune", [(False, False), (True, False), (True, True)])
+def test_commit_plan_mixed_sizes(use_graph, autotune):
+    """LDE + Merkle over columns of mixed sizes in one plan; every layer equals the
+    oracle's tree over the oracle's LDE, and graph replays reproduce it."""
+    blow = 1
+    logs = [15, 3, 10, 15, 1, 10, 10, 14, 6, 12, 15, 2]
+    m = max(logs) + blow
+    root = S.canonic_half_coset(m)
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << l, 8100 + i) for i, l in enumerate(logs)]
+    src = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    coeffs = [S.BaseFieldVec.uninitialized(1 << l) for l in logs]
+    ldes = [S.BaseFieldVec.uninitialized(1 << (l + blow)) for l in logs]
+    plan = S.CommitPlan(src, coeffs, ldes, logs, blow, tw, itw, 1 << root.log_size, autotune=autotune)
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    want_lde = []
+    for e, l in zip(evals, logs):
+        d = S.canonic_half_coset(l + blow)
+        t, it = O.precompute_twiddles(d.initial_index, d.step_size, d.log_size)
+        want_lde.append(O.lde_columns([e], blow, t, it)[1][0])
+    want = O.merkle_commit(want_lde)
+    for rep in range(3):
+        for l in ldes:
+            S.fill_uniform(l, 99 + rep)  # clobber, so each run must rewrite everything
+        plan.run(use_graph)
+        S.synchronize()
+        for i in range(len(logs)):
+            assert (ldes[i].to_numpy() == want_lde[i]).all(), f"rep {rep} column {i}"
+        for k in range(plan.max_log + 1):
+            assert (plan.layers[k].to_numpy().reshape(-1) == want[k]).all(), f"rep {rep} layer {k}"
+        assert (plan.root() == want[0]).all()
+    plan.close()
