@@ -370,6 +370,11 @@ def bench_batched(args):
 
     ms_graph = _timed_steps(S, lambda: plan.run(True), args.steps, args.warmup)
     ms_stream = _timed_steps(S, lambda: plan.run(False), args.steps, args.warmup)
+    names = ["fft_inv_tile", "fft_fwd_tile", "fft_inv_contig", "fft_fwd_contig", "fft_inv_strided", "fft_fwd_strided",
+             "fft_lde_strided", "fft_tiny_inv", "fft_tiny_fwd", "merkle_leaf", "merkle_layer", "merkle_subtree"]
+    ks = _kernel_avg(S, lambda: plan.run(False), names)
+    for v in ks.values():
+        v["ms_per_step"] = round(v["avg_ms"] * v["launches_per_step"], 4)
     root_batched = plan.root()
     layer0 = plan.layers[0].to_numpy().reshape(-1)
 
@@ -414,6 +419,7 @@ def bench_batched(args):
         "batched_graph_ms": round(ms_graph, 4), "batched_stream_ms": round(ms_stream, 4),
         "plain_per_column_ms": round(ms_plain, 3), "speedup_vs_plain": round(ms_plain / ms_graph, 1),
         "plan_create_and_autotune_s": round(plan_s, 3), "columns": C,
+        "kernels_stream_mode": ks,
         "columns_per_log": {str(k): logs.count(k) for k in range(10, 23)},
         "root": "".join(f"{w:08x}" for w in root_batched),
         "plain_root_matches_batched": bool((plain_root == layer0).all()),

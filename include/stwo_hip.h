@@ -165,7 +165,8 @@ enum {
   STWO_HIP_ERR_ARG = 1,    /* invalid argument (size, null, not a power of two, ...) */
   STWO_HIP_ERR_HIP = 2,    /* HIP runtime error */
   STWO_HIP_ERR_NOMEM = 3,  /* device allocation failed */
-  STWO_HIP_ERR_NODEV = 4   /* no gfx950 device / library not initialised */
+  STWO_HIP_ERR_NODEV = 4,  /* no gfx950 device / library not initialised */
+  STWO_HIP_NOT_READY = 5   /* not an error: nothing finished yet (server wait) */
 };
 
 /* Initialise on `device` (idempotent).  Every other entry point initialises device 0 lazily. */
@@ -259,6 +260,41 @@ int stwo_hip_commit_plan_create(StwoCommitPlan **plan, uint32_t n_cols, const ui
                                 uint64_t tree_len, StwoBlake2sHash *const *layers, uint32_t max_log, int autotune);
 int stwo_hip_commit_plan_run(StwoCommitPlan *plan, int use_graph);
 int stwo_hip_commit_plan_destroy(StwoCommitPlan *plan);
+
+/* Proof of work (GrindOps<Blake2sChannel>::grind, which the reference delegates to the CPU
+ * SimdBackend, stwo_gpu_backend/src/backend.rs:14-18): the smallest nonce whose
+ * BLAKE2s-256(digest || lo32(nonce) || hi32(nonce)) starts (as a little-endian 64-bit
+ * value) with pow_bits zero bits.  digest: 8 host words. */
+int stwo_hip_grind(const uint32_t *digest, uint32_t pow_bits, uint64_t *nonce);
+
+/* Wide-Fibonacci prover (the WideFibComponent of stwo_gpu_backend/src/examples/wide_fibonacci.rs
+ * proved end to end on the device: trace LDE + commitment, composition, OODS samples, DEEP
+ * quotients, FRI, proof of work, decommitments).  The trace is generated on the device from
+ * `seed`.  The proof is a word array whose layout the header (first 8 words) describes. */
+typedef struct {
+  uint32_t n_cols;    /* >= 3 */
+  uint32_t log_n;     /* trace rows = 2^log_n, 4 .. 26 */
+  uint32_t n_queries; /* 1 .. 64 */
+  uint32_t pow_bits;  /* proof-of-work bits, <= 32 */
+  uint32_t last_log;  /* log size of the last FRI layer, 2 .. 10 */
+} StwoWideFibParams;
+int stwo_hip_wide_fib_proof_size(const StwoWideFibParams *params, uint64_t *proof_words, uint64_t *device_bytes);
+/* One proof on the library stream; returns when the proof is in `proof`. */
+int stwo_hip_wide_fib_prove(const StwoWideFibParams *params, uint64_t seed, uint32_t *proof,
+                            uint64_t proof_capacity_words);
+
+/* Proof server with continuous batching: jobs are admitted in arrival order while fewer than
+ * max_inflight run and their device footprints fit hbm_budget_bytes; each runs on its own
+ * stream and overlaps with the others.  wait() returns STWO_HIP_NOT_READY after timeout_us
+ * without a finished job; otherwise the job's id, status, proof (when it fits) and its
+ * submit / start / end times in seconds since the server was created. */
+typedef struct StwoProofServer StwoProofServer;
+int stwo_hip_server_create(StwoProofServer **server, uint64_t hbm_budget_bytes, uint32_t max_inflight);
+int stwo_hip_server_submit(StwoProofServer *server, const StwoWideFibParams *params, uint64_t seed,
+                           uint64_t *job_id);
+int stwo_hip_server_wait(StwoProofServer *server, uint32_t timeout_us, uint64_t *job_id, int *job_status,
+                         uint32_t *proof, uint64_t proof_capacity_words, uint64_t *proof_words, double *times);
+int stwo_hip_server_destroy(StwoProofServer *server);
 
 /* Timing on the library stream (HIP events). */
 int stwo_hip_timer_begin(void);

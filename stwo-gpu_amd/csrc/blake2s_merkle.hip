@@ -117,16 +117,19 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
 }
 
 
-// Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b
-// reads 2^depth children hashes of layer top+1 (prev) and produces 2^(depth-1)
-// nodes of layer `top` ... down to one node of layer top-depth+1.  outs[k] and
-// inj[k] are indexed by layer k.
+// Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b reads
+// 2^(w_log+1) children hashes of layer top+1 (prev), produces 2^w_log nodes of layer
+// `top` and halves them in LDS down to 2^(w_log-depth+1) nodes of layer top-depth+1,
+// i.e. it builds 2^(w_log-depth+1) adjacent subtrees side by side (w_log >= depth-1;
+// shallow runs are packed so the first layer still spans the whole block).  outs[k]
+// and inj[k] are indexed by layer k.
 constexpr int SUB_MAX_DEPTH = 11;  // 1024 nodes in the first produced layer
-__global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth,
+constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
+__global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
                                                       Hash* const* outs, const LayerCols* inj) {
-  __shared__ Hash s[1 << (SUB_MAX_DEPTH - 1)];
-  const uint32_t nodes0 = 1u << (depth - 1);
-  const uint32_t base0 = blockIdx.x * nodes0;
+  __shared__ Hash s[1 << SUB_MAX_W_LOG];
+  const uint32_t nodes0 = 1u << w_log;
+  const uint32_t base0 = blockIdx.x << w_log;
   for (uint32_t d = 0; d < depth; d++) {
     const uint32_t layer = top - d;
     const uint32_t cnt = nodes0 >> d;
@@ -221,9 +224,11 @@ void MerklePlan::enqueue() const {
         layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev, outs[k]);
       });
     } else {
-      const uint32_t blocks = 1u << (s.top + 1 - s.depth);
+      // at least 256 nodes of layer top per block (or the whole layer when it is smaller)
+      const uint32_t w_log = std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(8, s.top));
+      const uint32_t blocks = 1u << (s.top - w_log);
       timed("merkle_subtree", [&] {
-        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, douts.as<Hash* const>(),
+        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts.as<Hash* const>(),
                                                dlcs.as<const LayerCols>());
       });
     }

@@ -70,7 +70,79 @@ __global__ __launch_bounds__(NT) void reduce_pass(const qm31* in, uint32_t g, Fa
   if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
+// ---- many columns of one size at one point held in device memory ----
+__global__ void factors_kernel(const qm31* __restrict__ point, uint32_t L, Factors* F) {
+  qm31 x = point[0];
+  F->f[L - 1] = point[1];
+  for (uint32_t i = 2; i <= L; i++) {
+    F->f[L - i] = x;
+    x = qsub(qadd(qsqr(x), qsqr(x)), qone());
+  }
+}
+
+// block (b, col) folds coefficients [b 2^cb, (b+1) 2^cb) of column col
+__global__ __launch_bounds__(NT) void first_pass_cols(const uint32_t* const* __restrict__ cols, uint32_t L,
+                                                      uint32_t cb, const Factors* __restrict__ F, qm31* partial) {
+  __shared__ qm31 s[1 << (CB - 1)];
+  const uint32_t* c = cols[blockIdx.y];
+  const uint32_t pairs = 1u << (cb - 1);
+  const size_t base = (size_t)blockIdx.x << cb;
+  const qm31 fy = F->f[L - 1];
+  for (uint32_t i = threadIdx.x; i < pairs; i += NT) {
+    uint32_t c0 = c[base + 2 * i], c1 = c[base + 2 * i + 1];
+    qm31 v = qscale(fy, c1);
+    v.x.a = add(v.x.a, c0);
+    s[i] = v;
+  }
+  __syncthreads();
+  tree_reduce(s, pairs, *F, (int)L - 2);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g, const Factors* __restrict__ F,
+                                                      int fi, qm31* out) {
+  __shared__ qm31 s[1 << (CB - 1)];
+  const uint32_t cnt = 1u << g;
+  for (uint32_t i = threadIdx.x; i < cnt; i += NT) s[i] = in[((size_t)blockIdx.x << g) + i];
+  __syncthreads();
+  tree_reduce(s, cnt, *F, fi);
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
 }  // namespace eap
+
+size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L) {
+  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
+  return 512 + 2 * sizeof(qm31) * (size_t)n_cols * (1ull << (L - cb));
+}
+
+// d_out[c] = value at d_point (x, y) of the polynomial with coefficients d_cols[c] (2^L each).
+void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
+                                 qm31* d_out, void* d_scratch) {
+  require(L >= 1 && L <= 31 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
+  eap::Factors* F = (eap::Factors*)d_scratch;
+  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
+  uint32_t blocks = 1u << (L - cb);
+  qm31* pa = (qm31*)((char*)d_scratch + 512);
+  qm31* pb = pa + (size_t)n_cols * blocks;
+  timed("eval_at_point", [&] {
+    eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
+    eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
+                                                                           blocks == 1 ? d_out : pa);
+  });
+  int fi = (int)(L - cb) - 1;
+  while (blocks > 1) {
+    const uint32_t g = std::min<uint32_t>(log2_exact(blocks), eap::CB - 1);
+    const uint32_t nb = blocks >> g;
+    qm31* dst = nb == 1 ? d_out : pb;
+    timed("eval_at_point", [&] {
+      eap::reduce_pass_dev<<<nb * n_cols, eap::NT, 0, ctx().cur>>>(pa, g, F, fi, dst);
+    });
+    std::swap(pa, pb);
+    fi -= (int)g;
+    blocks = nb;
+  }
+}
 
 qm31 eval_at_point_sync(const uint32_t* coeffs, uint64_t n, qm31 px, qm31 py) {
   require(is_pow2(n) && n <= (1ull << 31), "eval_at_point: size must be a power of two");

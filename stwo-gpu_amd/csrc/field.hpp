@@ -106,6 +106,7 @@ __host__ __device__ __forceinline__ qm31 qpow(qm31 p, uint64_t e) {
   }
   return r;
 }
+__host__ __device__ __forceinline__ bool qis_zero(qm31 p) { return !(p.x.a | p.x.b | p.y.a | p.y.b); }
 __host__ __device__ __forceinline__ bool qeq(qm31 p, qm31 q) {
   return p.x.a == q.x.a && p.x.b == q.x.b && p.y.a == q.y.a && p.y.b == q.y.b;
 }
@@ -139,6 +140,20 @@ __host__ __device__ __forceinline__ uint32_t circle_domain_index(uint32_t hi, ui
 }
 __host__ __device__ __forceinline__ uint32_t brev(uint32_t i, uint32_t bits) {
   return bits ? (__builtin_bitreverse32(i) >> (32 - bits)) : 0u;
+}
+
+// Synthetic uniform M31 stream: value i of (seed, offset) is the top 31 bits of the
+// SplitMix64 output for counter seed + (offset + i + 1) * golden gamma, with 2^31 - 1
+// mapped to the low 31 bits (then to 0 if still 2^31 - 1).
+__host__ __device__ __forceinline__ uint32_t uniform_m31(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  uint32_t v = (uint32_t)(z >> 33);
+  if (v == P) v = (uint32_t)(z & P);
+  if (v == P) v = 0;
+  return v;
 }
 
 // SoA secure-column access

@@ -28,20 +28,25 @@ __device__ __forceinline__ void st(const Soa4& s, size_t i, qm31 v) {
   s.c[3][i] = v.y.b;
 }
 
+// alpha_dev (optional): read alpha from device memory (a challenge drawn on the device)
 __global__ __launch_bounds__(256) void fold_line_kernel(const uint32_t* __restrict__ itw, uint32_t half_n,
-                                                        Soa4 src, qm31 alpha, Soa4 dst) {
+                                                        Soa4 src, qm31 alpha, const qm31* alpha_dev, Soa4 dst) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= half_n) return;
+  if (alpha_dev) alpha = *alpha_dev;
   uint32_t xi = itw[i];
   qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
   qm31 f0 = qadd(a, b), f1 = qscale(qsub(a, b), xi);
   st(dst, i, qadd(f0, qmul(alpha, f1)));
 }
 
+// accumulate = false: dst is overwritten with the fold (as if it held zeros)
 __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __restrict__ tw, uint32_t half_n,
-                                                          Soa4 src, qm31 alpha, qm31 alpha_sq, Soa4 dst) {
+                                                          Soa4 src, qm31 alpha, const qm31* alpha_dev,
+                                                          bool accumulate, Soa4 dst) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= half_n) return;
+  if (alpha_dev) alpha = *alpha_dev;
   uint32_t yi;
   if (half_n >= 4) {
     // circle twiddle i: [y, -y, -x, x] over pairs (x, y) of the first line level
@@ -58,8 +63,9 @@ __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __rest
   }
   qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
   qm31 f0 = qadd(a, b), f1 = qscale(qsub(a, b), yi);
-  qm31 prev = ld(dst, i);
-  st(dst, i, qadd(qmul(prev, alpha_sq), qadd(f0, qmul(alpha, f1))));
+  qm31 r = qadd(f0, qmul(alpha, f1));
+  if (accumulate) r = qadd(qmul(ld(dst, i), qsqr(alpha)), r);
+  st(dst, i, r);
 }
 
 static Soa4 soa(uint32_t* const* p) { return Soa4{{p[0], p[1], p[2], p[3]}}; }
@@ -67,22 +73,21 @@ static Soa4 soa(uint32_t* const* p) { return Soa4{{p[0], p[1], p[2], p[3]}}; }
 }  // namespace fri
 
 void fold_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src, qm31 alpha,
-                     uint32_t* const* dst) {
+                     uint32_t* const* dst, const qm31* alpha_dev) {
   require(n >= 2 && is_pow2(n) && n <= (1ull << 31), "fold_line: n must be a power of two >= 2");
   uint32_t half = (uint32_t)(n >> 1);
   timed("fold_line", [&] {
     fri::fold_line_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(itw + twiddle_offset, half, fri::soa(src),
-                                                                           alpha, fri::soa(dst));
+                                                                           alpha, alpha_dev, fri::soa(dst));
   });
 }
 void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src,
-                                 qm31 alpha, uint32_t* const* dst) {
+                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate) {
   require(n >= 2 && is_pow2(n) && n <= (1ull << 31), "fold_circle_into_line: n must be a power of two >= 2");
   uint32_t half = (uint32_t)(n >> 1);
-  qm31 a2 = qmul(alpha, alpha);
   timed("fold_circle_into_line", [&] {
-    fri::fold_circle_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(itw + twiddle_offset, half,
-                                                                             fri::soa(src), alpha, a2, fri::soa(dst));
+    fri::fold_circle_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(
+        itw + twiddle_offset, half, fri::soa(src), alpha, alpha_dev, accumulate, fri::soa(dst));
   });
 }
 
@@ -94,7 +99,7 @@ extern "C" int stwo_hip_fold_line_async(const uint32_t* itwiddles, size_t twiddl
                                         uint32_t* const* eval_values, StwoQM31 alpha, uint32_t* const* folded_values) {
   return guarded([&] {
     fold_line_async(itwiddles, twiddle_offset, n, eval_values, qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}},
-                    folded_values);
+                    folded_values, nullptr);
   });
 }
 
@@ -103,7 +108,7 @@ extern "C" int stwo_hip_fold_circle_into_line_async(const uint32_t* itwiddles, s
                                                     uint32_t* const* folded_values) {
   return guarded([&] {
     fold_circle_into_line_async(itwiddles, twiddle_offset, n, eval_values,
-                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values);
+                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true);
   });
 }
 
@@ -111,7 +116,7 @@ extern "C" void fold_line(const uint32_t* gpu_domain, size_t twiddle_offset, siz
                           StwoQM31 alpha, uint32_t* const* folded_values) {
   guarded([&] {
     fold_line_async(gpu_domain, twiddle_offset, n, eval_values, qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}},
-                    folded_values);
+                    folded_values, nullptr);
     sync();
   });
 }
@@ -120,7 +125,7 @@ extern "C" void fold_circle_into_line(const uint32_t* gpu_domain, size_t twiddle
                                       uint32_t* const* eval_values, StwoQM31 alpha, uint32_t* const* folded_values) {
   guarded([&] {
     fold_circle_into_line_async(gpu_domain, twiddle_offset, n, eval_values,
-                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values);
+                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true);
     sync();
   });
 }

@@ -1,0 +1,69 @@
+// Proof-of-work grinding: GrindOps<Blake2sChannel>::grind, which the reference backend
+// delegates to stwo's SimdBackend on the CPU (stwo_gpu_backend/src/backend.rs:14-18).
+//
+// The smallest nonce whose BLAKE2s(digest || nonce) has pow_bits trailing zero bits
+// (the rule of channel.hpp).  One launch: lane t of the grid tests nonces t, t + W,
+// t + 2W, ... (W = grid size) and records hits with a 64-bit atomic minimum.  A lane
+// stops once the recorded minimum is below the next nonce it would test, so the
+// smallest hit is always tested (every hit recorded before it is larger) and the
+// result does not depend on scheduling.  Every lane also stops after max_rounds, so
+// the launch always drains; no hit within the bound is reported as an error.
+#include "channel.hpp"
+#include "runtime.hpp"
+
+namespace sh {
+namespace grind {
+
+constexpr uint32_t NT = 256;
+constexpr uint32_t BLOCKS = 2048;  // 8 per CU on 256 CUs
+constexpr uint64_t WIDTH = (uint64_t)NT * BLOCKS;
+
+__global__ __launch_bounds__(NT) void grind_kernel(const uint32_t* __restrict__ digest, uint32_t pow_bits,
+                                                   uint32_t max_rounds, unsigned long long* best) {
+  uint32_t d[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) d[i] = digest[i];
+  const uint64_t t = blockIdx.x * (uint64_t)NT + threadIdx.x;
+  for (uint32_t r = 0; r < max_rounds; r++) {
+    const uint64_t nonce = t + r * WIDTH;
+    if (__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nonce) break;
+    if (pow_trailing_zeros(d, nonce) >= pow_bits) {
+      atomicMin(best, (unsigned long long)nonce);
+      break;
+    }
+  }
+}
+
+__global__ void init_kernel(unsigned long long* best) { *best = ~0ull; }
+
+}  // namespace grind
+
+// Result written to *d_nonce (device memory, ~0 when no nonce below max_rounds * WIDTH).
+void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce) {
+  require(pow_bits <= 40, "grind: at most 40 bits of work");
+  const uint64_t need = (1ull << pow_bits) * 64;  // ~e^-64 chance of no hit
+  const uint32_t rounds = (uint32_t)std::max<uint64_t>(1, (need + grind::WIDTH - 1) / grind::WIDTH);
+  timed("grind", [&] {
+    grind::init_kernel<<<1, 1, 0, ctx().cur>>>((unsigned long long*)d_nonce);
+    grind::grind_kernel<<<grind::BLOCKS, grind::NT, 0, ctx().cur>>>(d_digest, pow_bits, rounds,
+                                                                     (unsigned long long*)d_nonce);
+  });
+}
+
+}  // namespace sh
+
+using namespace sh;
+
+extern "C" int stwo_hip_grind(const uint32_t* digest, uint32_t pow_bits, uint64_t* nonce) {
+  return guarded([&] {
+    require(digest != nullptr && nonce != nullptr, "grind: null argument");
+    DevBuf d = upload(digest, 32);
+    DevBuf out(8);
+    grind_async(d.as<uint32_t>(), pow_bits, out.as<uint64_t>());
+    uint64_t r = 0;
+    hip_check(hipMemcpyAsync(&r, out.p, 8, hipMemcpyDeviceToHost, ctx().cur), "hipMemcpy");
+    sync();
+    require(r != ~0ull, "grind: no nonce found within the search bound");
+    *nonce = r;
+  });
+}

@@ -1,0 +1,459 @@
+// Wide-Fibonacci STARK prover built from the backend operations, in the shape of stwo's
+// prove() for the WideFibComponent (stwo_gpu_backend/src/examples/wide_fibonacci.rs:90-141
+// gives the composition step; trace LDE, commitments, OODS sampling, DEEP quotients and
+// FRI are the PolyOps / MerkleOps / QuotientOps / FriOps calls of poly.rs, blake2s.rs,
+// quotient.rs and fri.rs).  Statement: n_cols columns of 2^log_n rows with
+// col[i] = col[i-2]^2 + col[i-1]^2.
+//
+// Protocol (every challenge comes from the channel of channel.hpp):
+//  1. mix the statement (n_cols, log_n, n_queries, pow_bits, last_log, tag);
+//  2. trace T_i on CanonicCoset(log_n); interpolate, evaluate on CanonicCoset(log_n + 1)
+//     (blowup 2), commit the LDE columns (tree T), mix its root;
+//  3. alpha <- draw; composition E(p) = sum_i (T_i^2 + T_{i+1}^2 - T_{i+2}) alpha^(C-3-i) / Z(p)
+//     on the rows of CanonicCoset(log_n + 1), Z(p) = pi^(log_n - 1)(p.x); its four
+//     coordinate columns are interpolated there and evaluated on CanonicCoset(log_n + 2),
+//     committed (tree Q), root mixed;
+//  4. OODS point z <- draw_point; samples T_i(z) and E_k(z) (from the coefficients), mixed;
+//  5. beta <- draw, a_0 <- draw; F1 = quotients of the trace LDE at z (domain log_n + 1),
+//     F2 = quotients of the composition LDE at z (domain log_n + 2), random coefficient beta;
+//  6. FRI: line_1 = fold_circle(F2, a_0); for j = 1 .. n_line: commit line_j (tree L_j),
+//     mix its root, a_j <- draw, line_{j+1} = fold_line(line_j, a_j), and for j = 1 also
+//     fold_circle_into_line(line_2, F1, a_1); the last line (2^last_log values) is mixed
+//     and sent in the clear;
+//  7. proof of work (pow_bits), nonce mixed; n_queries positions P in [0, 2^(log_n+2));
+//  8. decommitments per query: trace rows P>>1 and its sibling, composition rows P and
+//     P^1, line j rows P>>j and its sibling, each with the Merkle siblings of every layer
+//     above the leaves.
+// Nothing returns to the host before the proof is complete: the channel runs in
+// single-lane kernels on device-resident state, and one gather kernel assembles the
+// proof in device memory before a single copy to pinned host memory.
+#include <array>
+#include <cstring>
+#include <map>
+
+#include "channel.hpp"
+#include "fft.hpp"
+#include "prover.hpp"
+
+namespace sh {
+
+// other modules
+void wide_fib_trace_async(uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_n, const uint64_t* d_seed);
+void vanishing_inverse_async(uint32_t* d_out, uint32_t log_ext, uint32_t log_n);
+void qm31_powers_async(const qm31* d_alpha, uint32_t n, uint32_t* d_powers);
+void wide_fib_quotients_async(uint32_t* const q[4], const uint32_t* const* d_trace, const uint32_t* powers,
+                              const uint32_t* dinv, uint32_t n_rows, uint32_t n_cols);
+size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L);
+void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
+                                 qm31* d_out, void* d_scratch);
+size_t quotient_single_point_scratch_bytes(uint32_t n_cols);
+void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
+                                             const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
+                                             const qm31* d_values, const qm31* d_rc, void* d_scratch,
+                                             uint32_t* const out[4]);
+void fold_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src, qm31 alpha,
+                     uint32_t* const* dst, const qm31* alpha_dev);
+void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src,
+                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate);
+void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce);
+
+namespace prover {
+
+constexpr uint32_t MAGIC = 0x50575453u;  // "STWP"
+constexpr uint32_t VERSION = 1;
+constexpr uint32_t STATEMENT_TAG = 0x57464942u;  // "BIFW"
+constexpr uint32_t MAX_QUERIES = 64;
+constexpr uint32_t MAX_ALPHAS = 32;
+
+struct JobBlock {
+  Channel ch;
+  uint32_t header[8];
+  uint64_t seed;
+  uint64_t nonce;
+  qm31 alpha;
+  qm31 z[2];
+  qm31 beta;
+  qm31 fri_alpha[MAX_ALPHAS];
+  uint32_t queries[MAX_QUERIES];
+};
+
+struct GatherOp {
+  const uint32_t* src;
+  uint32_t dst;
+  uint16_t words;
+  int16_t q;  // query whose position indexes src, or -1 for a fixed source
+  uint8_t shift, xr;
+  uint16_t pad;
+};
+
+// ---- channel steps (one lane) ----
+__global__ void k_begin(JobBlock* jb, uint64_t seed, Params p, uint32_t n_line) {
+  jb->seed = seed;
+  const uint32_t h[8] = {MAGIC, VERSION, p.n_cols, p.log_n, p.n_queries, p.pow_bits, p.last_log, n_line};
+  for (int i = 0; i < 8; i++) jb->header[i] = h[i];
+  jb->ch.reset();
+  const uint32_t st[6] = {p.n_cols, p.log_n, p.n_queries, p.pow_bits, p.last_log, STATEMENT_TAG};
+  jb->ch.mix(st, 6);
+}
+__global__ void k_mix_root_draw(JobBlock* jb, const merkle::Hash* root, qm31* dst) {
+  jb->ch.mix(root->s, 8);
+  *dst = jb->ch.draw_qm31();
+}
+__global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root) {
+  jb->ch.mix(root->s, 8);
+  jb->ch.draw_point(jb->z[0], jb->z[1]);
+}
+__global__ void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n) {
+  jb->ch.mix_qm31(samples, n);
+  jb->beta = jb->ch.draw_qm31();
+  jb->fri_alpha[0] = jb->ch.draw_qm31();
+}
+struct Soa4c {
+  const uint32_t* c[4];
+};
+__global__ void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
+  b2s::Hasher hs;
+  hs.words(jb->ch.digest, 8);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t w[4] = {last.c[0][i], last.c[1][i], last.c[2][i], last.c[3][i]};
+    hs.words(w, 4);
+  }
+  hs.finish(jb->ch.digest);
+  jb->ch.n_draws = 0;
+}
+__global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size) {
+  const uint32_t w[2] = {(uint32_t)jb->nonce, (uint32_t)(jb->nonce >> 32)};
+  jb->ch.mix(w, 2);
+  jb->ch.draw_queries(jb->queries, n_queries, log_size);
+}
+
+__global__ __launch_bounds__(256) void k_gather(const GatherOp* __restrict__ ops, uint32_t n_ops,
+                                                const JobBlock* __restrict__ jb, uint32_t* __restrict__ proof) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_ops) return;
+  const GatherOp op = ops[i];
+  const uint32_t idx = op.q < 0 ? 0u : ((jb->queries[op.q] >> op.shift) ^ op.xr);
+  const uint32_t* src = op.src + (size_t)idx * op.words;
+  for (uint32_t w = 0; w < op.words; w++) proof[op.dst + w] = src[w];
+}
+
+// ---- layout ----
+void validate(const Params& p) {
+  require(p.n_cols >= 3 && p.n_cols <= 4096, "wide fibonacci prover: n_cols must be in [3, 4096]");
+  require(p.last_log >= 2 && p.last_log <= 10, "wide fibonacci prover: last_log must be in [2, 10]");
+  require(p.log_n >= 4 && p.log_n >= p.last_log && p.log_n <= 26, "wide fibonacci prover: log_n out of range");
+  require(p.n_queries >= 1 && p.n_queries <= MAX_QUERIES, "wide fibonacci prover: n_queries must be in [1, 64]");
+  require(p.pow_bits <= 32, "wide fibonacci prover: pow_bits must be <= 32");
+}
+
+Layout::Layout(const Params& p) {
+  const uint64_t C = p.n_cols;
+  n_line = p.log_n + 1 - p.last_log;
+  uint64_t off = 8;
+  roots = off;
+  off += 8ull * (2 + n_line);
+  samples = off;
+  off += 4 * (C + 4);
+  last = off;
+  off += 4ull << p.last_log;
+  nonce = off;
+  off += 2;
+  queries = off;
+  off += p.n_queries;
+  per_query = off;
+  uint64_t o = 0;
+  q_trace_vals = o;
+  o += 2 * C;
+  q_trace_path = o;
+  o += 8ull * p.log_n;
+  q_comp_vals = o;
+  o += 8;
+  q_comp_path = o;
+  o += 8ull * (p.log_n + 1);
+  for (uint32_t j = 1; j <= n_line; j++) {
+    const uint32_t m = p.log_n + 2 - j;
+    q_line.push_back(o);
+    o += 8 + 8ull * (m - 1);
+  }
+  query_words = o;
+  total = per_query + p.n_queries * query_words;
+}
+
+namespace {
+
+uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
+
+// arena layout of one job (byte offsets)
+struct Arena {
+  uint64_t coeffs, lde, tree_t, dinv, powers, comp, comp_lde, tree_q, f1, f2, eap, quot, samples, jb, proof, end;
+  std::vector<uint64_t> line, tree_l;
+  explicit Arena(const Params& p, const Layout& L) {
+    const uint64_t C = p.n_cols, n = 1ull << p.log_n;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+      uint64_t at = o;
+      o += align256(bytes);
+      return at;
+    };
+    coeffs = take(4 * C * n);
+    lde = take(8 * C * n);
+    tree_t = take(32 * ((4 * n) - 1));  // layers 0 .. log_n + 1
+    dinv = take(8 * n);
+    powers = take(16 * (C - 2));
+    comp = take(4 * 4 * 2 * n);
+    comp_lde = take(4 * 4 * 4 * n);
+    tree_q = take(32 * ((8 * n) - 1));
+    f1 = take(4 * 4 * 2 * n);
+    f2 = take(4 * 4 * 4 * n);
+    for (uint32_t j = 1; j <= L.n_line + 1; j++) line.push_back(take(16ull << (p.log_n + 2 - j)));
+    for (uint32_t j = 1; j <= L.n_line; j++) tree_l.push_back(take(32 * ((2ull << (p.log_n + 2 - j)) - 1)));
+    eap = take(std::max(eval_at_point_columns_scratch_bytes(p.n_cols, p.log_n),
+                        eval_at_point_columns_scratch_bytes(4, p.log_n + 1)));
+    quot = take(quotient_single_point_scratch_bytes(std::max<uint32_t>(p.n_cols, 4)));
+    samples = take(16 * (C + 4));
+    jb = take(sizeof(JobBlock));
+    proof = take(4 * L.total);
+    end = o;
+  }
+};
+
+// layers[k] of a tree whose layers 0 .. max_log are packed from `base`
+std::vector<merkle::Hash*> tree_layers(char* base, uint32_t max_log) {
+  std::vector<merkle::Hash*> v(max_log + 1);
+  uint64_t o = 0;
+  for (uint32_t k = 0; k <= max_log; k++) {
+    v[k] = (merkle::Hash*)(base + o);
+    o += 32ull << k;
+  }
+  return v;
+}
+
+// pinned host buffers for finished proofs, reused by capacity
+std::multimap<size_t, void*> g_pinned_free;
+std::map<void*, size_t> g_pinned_cap;
+void* take_pinned(size_t bytes) {
+  auto it = g_pinned_free.lower_bound(bytes);
+  if (it != g_pinned_free.end() && it->first <= 2 * bytes + 4096) {
+    void* p = it->second;
+    g_pinned_free.erase(it);
+    return p;
+  }
+  void* p = nullptr;
+  hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+  g_pinned_cap[p] = bytes;
+  return p;
+}
+void give_pinned(void* p) { g_pinned_free.emplace(g_pinned_cap.at(p), p); }
+
+struct StreamScope {
+  Context& x;
+  hipStream_t saved;
+  StreamScope(hipStream_t s) : x(ctx()), saved(ctx().cur) { x.cur = s; }
+  ~StreamScope() { x.cur = saved; }
+};
+
+uint32_t half_initial(uint32_t log) { return 1u << (30 - log); }
+uint32_t half_step(uint32_t log) { return (1u << (32 - log)) & 0x7fffffffu; }
+
+}  // namespace
+
+uint64_t footprint_bytes(const Params& p) {
+  Layout L(p);
+  Arena A(p, L);
+  return A.end + 64 * 1024 * (uint64_t)(3 + L.n_line);  // plus plan tables
+}
+
+Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream(s) {
+  StreamScope scope(stream);
+  const uint32_t C = p.n_cols, ln = p.log_n;
+  const uint64_t n = 1ull << ln;
+
+  // one twiddle tree (CanonicCoset(log_n + 2)) serves every domain of the proof
+  {
+    const uint32_t lt = ln + 2;
+    int st = stwo_hip_twiddles(half_initial(lt), half_step(lt), lt - 1, &tw, &itw);
+    if (st != STWO_HIP_OK) fail(st, "wide fibonacci prover: twiddles");
+    tree_len = 1ull << (lt - 1);
+  }
+
+  Arena A(p, L);
+  arena = DevBuf(A.end);
+  char* base = arena.as<char>();
+  coeffs = (uint32_t*)(base + A.coeffs);
+  lde = (uint32_t*)(base + A.lde);
+  dinv = (uint32_t*)(base + A.dinv);
+  powers = (uint32_t*)(base + A.powers);
+  comp = (uint32_t*)(base + A.comp);
+  comp_lde = (uint32_t*)(base + A.comp_lde);
+  f1 = (uint32_t*)(base + A.f1);
+  f2 = (uint32_t*)(base + A.f2);
+  for (uint64_t o : A.line) line.push_back((uint32_t*)(base + o));
+  eap_scratch = base + A.eap;
+  quot_scratch = base + A.quot;
+  jb = (JobBlock*)(base + A.jb);
+  proof = (uint32_t*)(base + A.proof);
+  samples = (qm31*)(base + A.samples);
+
+  // device pointer tables: coeffs | lde | comp | comp_lde
+  std::vector<const uint32_t*> t;
+  for (uint32_t i = 0; i < C; i++) t.push_back(coeffs + i * n);
+  for (uint32_t i = 0; i < C; i++) t.push_back(lde + i * 2 * n);
+  for (uint32_t k = 0; k < 4; k++) t.push_back(comp + k * 2 * n);
+  for (uint32_t k = 0; k < 4; k++) t.push_back(comp_lde + k * 4 * n);
+  tables_ = upload(t.data(), t.size() * sizeof(void*));
+  const uint32_t* const* tb = tables_.as<const uint32_t*>();
+  t_coeffs = tb;
+  t_lde = tb + C;
+  t_comp = tb + 2 * C;
+  t_comp_lde = tb + 2 * C + 4;
+
+  // Merkle plans
+  layers_t = tree_layers(base + A.tree_t, ln + 1);
+  layers_q = tree_layers(base + A.tree_q, ln + 2);
+  {
+    std::vector<uint32_t> logs(C, ln + 1);
+    tree_t = std::make_unique<merkle::MerklePlan>(C, t.data() + C, logs.data(), layers_t.data(), ln + 1);
+    std::vector<uint32_t> l4(4, ln + 2);
+    tree_q = std::make_unique<merkle::MerklePlan>(4, t.data() + 2 * C + 4, l4.data(), layers_q.data(), ln + 2);
+  }
+  for (uint32_t j = 1; j <= L.n_line; j++) {
+    const uint32_t m = ln + 2 - j;
+    layers_l.push_back(tree_layers(base + A.tree_l[j - 1], m));
+    const uint32_t* cols[4];
+    for (int k = 0; k < 4; k++) cols[k] = line[j - 1] + ((uint64_t)k << m);
+    const uint32_t l4[4] = {m, m, m, m};
+    tree_l.push_back(std::make_unique<merkle::MerklePlan>(4, cols, l4, layers_l.back().data(), m));
+  }
+
+  // proof assembly table
+  std::vector<GatherOp> g;
+  auto fixed = [&](const void* src, uint64_t dst, uint32_t words) {
+    for (uint32_t w = 0; w < words; w += 16)
+      g.push_back(GatherOp{(const uint32_t*)src + w, (uint32_t)(dst + w), (uint16_t)std::min<uint32_t>(16, words - w),
+                           -1, 0, 0, 0});
+  };
+  auto indexed = [&](const void* src, uint64_t dst, uint32_t words, uint32_t q, uint32_t shift, uint32_t xr) {
+    g.push_back(GatherOp{(const uint32_t*)src, (uint32_t)dst, (uint16_t)words, (int16_t)q, (uint8_t)shift,
+                         (uint8_t)xr, 0});
+  };
+  // siblings of layers leaf_log - 1 .. 1 for a leaf position (query position >> shift0)
+  auto path = [&](const std::vector<merkle::Hash*>& layers, uint32_t leaf_log, uint64_t dst, uint32_t q,
+                  uint32_t shift0) {
+    for (uint32_t k = leaf_log - 1, i = 0; k >= 1; k--, i++) indexed(layers[k], dst + 8 * i, 8, q, shift0 + leaf_log - k, 1);
+  };
+  fixed(jb->header, 0, 8);
+  fixed(layers_t[0], L.roots, 8);
+  fixed(layers_q[0], L.roots + 8, 8);
+  for (uint32_t j = 1; j <= L.n_line; j++) fixed(layers_l[j - 1][0], L.roots + 8 * (1 + j), 8);
+  fixed(base + A.samples, L.samples, 4 * (C + 4));
+  {
+    const uint32_t* last = line[L.n_line];
+    const uint32_t nl = 1u << p.last_log;
+    for (uint32_t v = 0; v < nl; v++)
+      for (uint32_t k = 0; k < 4; k++) fixed(last + (uint64_t)k * nl + v, L.last + 4 * v + k, 1);
+  }
+  fixed(&jb->nonce, L.nonce, 2);
+  fixed(jb->queries, L.queries, p.n_queries);
+  for (uint32_t q = 0; q < p.n_queries; q++) {
+    const uint64_t qb = L.per_query + q * L.query_words;
+    for (uint32_t s = 0; s < 2; s++)
+      for (uint32_t i = 0; i < C; i++) indexed(lde + i * 2 * n, qb + L.q_trace_vals + s * C + i, 1, q, 1, s);
+    path(layers_t, ln + 1, qb + L.q_trace_path, q, 1);
+    for (uint32_t s = 0; s < 2; s++)
+      for (uint32_t k = 0; k < 4; k++) indexed(comp_lde + k * 4 * n, qb + L.q_comp_vals + 4 * s + k, 1, q, 0, s);
+    path(layers_q, ln + 2, qb + L.q_comp_path, q, 0);
+    for (uint32_t j = 1; j <= L.n_line; j++) {
+      const uint32_t m = ln + 2 - j;
+      for (uint32_t s = 0; s < 2; s++)
+        for (uint32_t k = 0; k < 4; k++)
+          indexed(line[j - 1] + ((uint64_t)k << m), qb + L.q_line[j - 1] + 4 * s + k, 1, q, j, s);
+      path(layers_l[j - 1], m, qb + L.q_line[j - 1] + 8, q, j);
+    }
+  }
+  n_ops = (uint32_t)g.size();
+  ops = upload(g.data(), g.size() * sizeof(GatherOp));
+
+  host_proof = take_pinned(4 * L.total);
+  hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
+}
+
+Job::~Job() {
+  // the caller has waited for `done`
+  if (done) (void)hipEventDestroy(done);
+  if (host_proof) give_pinned(host_proof);
+  StreamScope scope(stream);
+  tree_t.reset();
+  tree_q.reset();
+  tree_l.clear();
+  ops = DevBuf();
+  tables_ = DevBuf();
+  arena = DevBuf();
+}
+
+void Job::enqueue(uint64_t seed) {
+  StreamScope scope(stream);
+  const uint32_t C = p.n_cols, ln = p.log_n;
+  const uint64_t n = 1ull << ln;
+
+  timed("channel", [&] { k_begin<<<1, 1, 0, stream>>>(jb, seed, p, L.n_line); });
+  wide_fib_trace_async((uint32_t* const*)t_coeffs, C, ln, &jb->seed);
+  fft::lde_same_size(t_coeffs, (uint32_t* const*)t_coeffs, (uint32_t* const*)t_lde, C, ln, 1, tw, itw, tree_len,
+                     fft::lde_variant_for(ln, 1, C));
+  tree_t->enqueue();
+  timed("channel", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
+
+  qm31_powers_async(&jb->alpha, C - 2, powers);
+  vanishing_inverse_async(dinv, ln + 1, ln);
+  uint32_t* q4[4] = {comp, comp + 2 * n, comp + 4 * n, comp + 6 * n};
+  wide_fib_quotients_async(q4, t_lde, powers, dinv, (uint32_t)(2 * n), C);
+  fft::lde_same_size(t_comp, (uint32_t* const*)t_comp, (uint32_t* const*)t_comp_lde, 4, ln + 1, 1, tw, itw, tree_len,
+                     fft::lde_variant_for(ln + 1, 1, 4));
+  tree_q->enqueue();
+  timed("channel", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
+
+  eval_at_point_columns_async(t_coeffs, C, ln, jb->z, samples, eap_scratch);
+  eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch);
+  timed("channel", [&] { k_mix_samples_draw<<<1, 1, 0, stream>>>(jb, samples, C + 4); });
+
+  uint32_t* f1c[4] = {f1, f1 + 2 * n, f1 + 4 * n, f1 + 6 * n};
+  uint32_t* f2c[4] = {f2, f2 + 4 * n, f2 + 8 * n, f2 + 12 * n};
+  accumulate_quotients_single_point_async(half_initial(ln + 1), half_step(ln + 1), ln + 1, t_lde, C, jb->z, samples,
+                                          &jb->beta, quot_scratch, f1c);
+  accumulate_quotients_single_point_async(half_initial(ln + 2), half_step(ln + 2), ln + 2, t_comp_lde, 4, jb->z,
+                                          samples + C, &jb->beta, quot_scratch, f2c);
+
+  auto coords = [&](uint32_t* c, uint32_t m) {
+    std::array<uint32_t*, 4> r;
+    for (int k = 0; k < 4; k++) r[k] = c + ((uint64_t)k << m);
+    return r;
+  };
+  auto l1 = coords(line[0], ln + 1);
+  fold_circle_into_line_async(itw, tree_len - (1ull << (ln + 1)), 4 * n, f2c, qzero(), l1.data(), &jb->fri_alpha[0],
+                              false);
+  for (uint32_t j = 1; j <= L.n_line; j++) {
+    const uint32_t m = ln + 2 - j;
+    tree_l[j - 1]->enqueue();
+    timed("channel", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
+    auto src = coords(line[j - 1], m), dst = coords(line[j], m - 1);
+    fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j]);
+    if (j == 1)
+      fold_circle_into_line_async(itw, tree_len - (1ull << ln), 2 * n, f1c, qzero(), dst.data(), &jb->fri_alpha[1],
+                                  true);
+  }
+  {
+    const uint32_t nl = 1u << p.last_log;
+    const uint32_t* last = line[L.n_line];
+    Soa4c s{{last, last + nl, last + 2 * nl, last + 3 * nl}};
+    timed("channel", [&] { k_mix_last<<<1, 1, 0, stream>>>(jb, s, nl); });
+  }
+  grind_async(jb->ch.digest, p.pow_bits, &jb->nonce);
+  timed("channel", [&] { k_mix_nonce_queries<<<1, 1, 0, stream>>>(jb, p.n_queries, ln + 2); });
+  timed("proof_gather", [&] {
+    k_gather<<<ceil_div(n_ops, 256), 256, 0, stream>>>(ops.as<GatherOp>(), n_ops, jb, proof);
+  });
+  hip_check(hipMemcpyAsync(host_proof, proof, 4 * L.total, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+  hip_check(hipEventRecord(done, stream), "hipEventRecord");
+}
+
+}  // namespace prover
+}  // namespace sh

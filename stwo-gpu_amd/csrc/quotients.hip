@@ -87,7 +87,62 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
   o3[row] = acc.y.b;
 }
 
+// One batch (every column sampled once at one point) built on the device from a
+// point, sample values and random coefficient that live in device memory: lane j
+// takes samples j, j + 256, ...; the per-batch sums are reduced in LDS.
+constexpr uint32_t SB_NT = 256;
+__global__ __launch_bounds__(SB_NT) void single_batch_kernel(const qm31* __restrict__ point,
+                                                             const qm31* __restrict__ values, uint32_t count,
+                                                             const qm31* __restrict__ rc_dev, Sample* ss, Batch* bs) {
+  __shared__ qm31 sa[SB_NT], sb[SB_NT];
+  const qm31 px = point[0], py = point[1], rc = *rc_dev;
+  const qm31 c = qsub(qm31{py.x, cneg(py.y)}, py);
+  qm31 la = qzero(), lb = qzero();
+  for (uint32_t j = threadIdx.x; j < count; j += SB_NT) {
+    const qm31 alpha = qpow(rc, j + 1);
+    const qm31 v = values[j];
+    const qm31 a = qsub(qm31{v.x, cneg(v.y)}, v);
+    const qm31 bb = qsub(qmul(v, c), qmul(a, py));
+    la = qadd(la, qmul(alpha, a));
+    lb = qadd(lb, qmul(alpha, bb));
+    ss[j] = Sample{j, 0, alpha};
+  }
+  sa[threadIdx.x] = la;
+  sb[threadIdx.x] = lb;
+  __syncthreads();
+  for (uint32_t h = SB_NT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sa[threadIdx.x] = qadd(sa[threadIdx.x], sa[threadIdx.x + h]);
+      sb[threadIdx.x] = qadd(sb[threadIdx.x], sb[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bs[0] = Batch{0, count, c, sa[0], sb[0], qpow(rc, count), px.x, py.x, px.y, py.y};
+}
+
 }  // namespace quot
+
+size_t quotient_single_point_scratch_bytes(uint32_t n_cols) {
+  return ((sizeof(quot::Sample) * std::max<uint32_t>(n_cols, 1) + 255) & ~(size_t)255) + sizeof(quot::Batch);
+}
+
+// accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
+// the point (x, y), the sample values and the random coefficient read from device memory.
+void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
+                                             const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
+                                             const qm31* d_values, const qm31* d_rc, void* d_scratch,
+                                             uint32_t* const out[4]) {
+  require(log_size >= 1 && log_size <= 31 && n_cols >= 1, "accumulate_quotients: bad sizes");
+  quot::Sample* ss = (quot::Sample*)d_scratch;
+  quot::Batch* bs = (quot::Batch*)((char*)d_scratch + ((sizeof(quot::Sample) * n_cols + 255) & ~(size_t)255));
+  timed("quotient_constants", [&] {
+    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_point, d_values, n_cols, d_rc, ss, bs);
+  });
+  timed("accumulate_quotients", [&] {
+    quot::accumulate_kernel<<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
+        hi, hs, log_size, d_cols, ss, bs, 1, out[0], out[1], out[2], out[3]);
+  });
+}
 
 // points: n_batches x StwoCirclePointQM31 (host); indexes/values flattened in batch order (host).
 void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, const uint32_t* const* d_cols,

@@ -71,7 +71,12 @@ Context& ctx() { return g_ctx; }
 
 static void resolve_timings() {
   Context& c = g_ctx;
+  std::vector<TimedLaunch> later;
   for (auto& t : c.pending) {
+    if (hipEventQuery(t.stop) != hipSuccess) {  // launched on another stream, still running
+      later.push_back(t);
+      continue;
+    }
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, t.start, t.stop), "hipEventElapsedTime");
     auto& s = c.stats[t.name];
@@ -80,7 +85,7 @@ static void resolve_timings() {
     c.event_pool.push_back(t.start);
     c.event_pool.push_back(t.stop);
   }
-  c.pending.clear();
+  c.pending.swap(later);
 }
 
 void sync() {
@@ -120,9 +125,12 @@ size_t Pool::round(size_t b) {
 }
 void* Pool::alloc(size_t bytes) {
   size_t r = round(bytes);
-  auto it = cache_.lower_bound(r);
-  if (it != cache_.end() && it->first <= r + r / 4) {  // close fit only
-    void* p = it->second;
+  const hipStream_t cur = g_ctx.cur;
+  for (auto it = cache_.lower_bound(r); it != cache_.end() && it->first <= r + r / 4; ++it) {  // close fit only
+    Block& b = it->second;
+    if (b.stream != cur && hipEventQuery(b.done) != hipSuccess) continue;
+    void* p = b.p;
+    g_ctx.event_pool.push_back(b.done);
     cached_ -= it->first;
     live_[p] = it->first;
     cache_.erase(it);
@@ -146,13 +154,18 @@ void Pool::free(void* p) {
   if (!p) return;
   auto it = live_.find(p);
   if (it == live_.end()) fail(STWO_HIP_ERR_ARG, "free of a pointer the library did not allocate (%p)", p);
-  cache_.emplace(it->second, p);
+  Block b{p, g_ctx.cur, take_event()};
+  hip_check(hipEventRecord(b.done, b.stream), "hipEventRecord");
+  cache_.emplace(it->second, b);
   cached_ += it->second;
   live_.erase(it);
 }
 void Pool::release_cached() {
-  hip_check(hipStreamSynchronize(g_ctx.stream), "hipStreamSynchronize");
-  for (auto& kv : cache_) hip_check(hipFree(kv.second), "hipFree");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  for (auto& kv : cache_) {
+    hip_check(hipFree(kv.second.p), "hipFree");
+    g_ctx.event_pool.push_back(kv.second.done);
+  }
   cache_.clear();
   cached_ = 0;
 }
@@ -186,16 +199,7 @@ DevBuf upload(const void* host, size_t bytes) {
 __global__ void fill_uniform_kernel(uint32_t* dst, uint64_t n, uint64_t seed, uint64_t offset) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (offset + i + 1);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    uint32_t v = (uint32_t)(z >> 33);
-    if (v == P) v = (uint32_t)(z & P);
-    if (v == P) v = 0;
-    dst[i] = v;
-  }
+  for (; i < n; i += stride) dst[i] = uniform_m31(seed, offset + i);
 }
 
 // plain 16-byte copy (column copies; also the reference kernel for HBM counter calibration)

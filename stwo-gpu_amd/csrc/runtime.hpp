@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <map>
 #include <mutex>
@@ -30,20 +31,26 @@ inline void require(bool ok, const char* fmt, const char* detail = "") {
 }
 
 // ---- caching device allocator --------------------------------------------
-// All library work runs on one stream, so a block returned to the cache can be
-// handed out again at once: later work on the stream is ordered after the
-// block's previous users.
+// A freed block remembers the stream it was last used on and an event recorded there
+// at the time of the free.  It is handed out again at once to work on the same stream
+// (later work on that stream is ordered after its previous users), and to work on
+// another stream only once that event has completed.
 class Pool {
  public:
-  void* alloc(size_t bytes);
-  void free(void* p);
-  void release_cached();  // synchronises the stream, then returns cached blocks to HIP
+  void* alloc(size_t bytes);  // for work launched on ctx().cur
+  void free(void* p);         // every user of p is already enqueued on ctx().cur
+  void release_cached();      // synchronises the device, then returns cached blocks to HIP
   size_t cached_bytes() const { return cached_; }
 
  private:
+  struct Block {
+    void* p;
+    hipStream_t stream;
+    hipEvent_t done;
+  };
   static size_t round(size_t b);
   std::unordered_map<void*, size_t> live_;
-  std::multimap<size_t, void*> cache_;
+  std::multimap<size_t, Block> cache_;
   size_t cached_ = 0;
 };
 

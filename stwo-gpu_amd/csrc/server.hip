@@ -1,0 +1,268 @@
+// Proof serving with continuous batching.
+//
+// Requests (wide-Fibonacci statements + trace seeds) queue in arrival order.  A worker
+// thread admits the head of the queue as soon as a slot stream is free and the job's
+// device footprint fits the HBM budget next to the jobs in flight, allocates its buffers
+// and enqueues the whole proof on the slot's stream (no host round trip inside a proof,
+// see prover.hip), so jobs that arrive while others run overlap with them on the
+// device.  Finished jobs (their completion event has fired) release memory and slot at
+// once, which admits the next waiting job.  The twiddle trees (one per log size) stay
+// cached on the device for every later job.
+//
+// A job larger than the whole budget is admitted alone.  Results are returned in
+// completion order with their submit / start / end times.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "prover.hpp"
+
+namespace sh {
+namespace serve {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t id;
+  prover::Params p;
+  uint64_t seed;
+  double t_submit;
+};
+struct Running {
+  Request r;
+  std::unique_ptr<prover::Job> job;
+  int slot;
+  uint64_t bytes;
+  double t_start;
+};
+struct Result {
+  uint64_t id;
+  int status;
+  std::vector<uint32_t> proof;
+  double t_submit, t_start, t_end;
+};
+
+struct Server {
+  uint64_t budget;
+  uint32_t max_inflight;
+  Clock::time_point t0 = Clock::now();
+  std::vector<hipStream_t> streams;
+  std::vector<bool> slot_busy;
+
+  std::mutex m;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> queue;
+  std::deque<Result> results;
+  uint64_t next_id = 1;
+  bool stopping = false;
+
+  std::vector<Running> running;  // worker thread only
+  uint64_t inflight_bytes = 0;
+  std::thread worker;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  void finish(Running& r, int status) {
+    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now()};
+    if (status == STWO_HIP_OK) {
+      const uint32_t* pr = r.job->result();
+      res.proof.assign(pr, pr + r.job->L.total);
+      // no proof-of-work nonce found within the search bound
+      if (pr[r.job->L.nonce] == 0xffffffffu && pr[r.job->L.nonce + 1] == 0xffffffffu) res.status = STWO_HIP_ERR_ARG;
+    }
+    {
+      auto g = lock();
+      r.job.reset();
+    }
+    slot_busy[r.slot] = false;
+    inflight_bytes -= r.bytes;
+    std::lock_guard<std::mutex> l(m);
+    results.push_back(std::move(res));
+    cv_done.notify_all();
+  }
+
+  void loop() {
+    for (;;) {
+      bool progressed = false;
+      // completions
+      for (size_t i = 0; i < running.size();) {
+        const hipError_t q = hipEventQuery(running[i].job->done);
+        if (q == hipErrorNotReady) {
+          i++;
+          continue;
+        }
+        finish(running[i], q == hipSuccess ? STWO_HIP_OK : STWO_HIP_ERR_HIP);
+        running.erase(running.begin() + i);
+        progressed = true;
+      }
+      // admissions, in arrival order
+      for (;;) {
+        Request r;
+        {
+          std::lock_guard<std::mutex> l(m);
+          if (queue.empty() || running.size() >= max_inflight) break;
+          const uint64_t need = prover::footprint_bytes(queue.front().p);
+          if (!running.empty() && inflight_bytes + need > budget) break;
+          r = queue.front();
+          queue.pop_front();
+        }
+        int slot = 0;
+        while (slot_busy[slot]) slot++;
+        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), now()};
+        int st = STWO_HIP_OK;
+        {
+          auto g = lock();
+          try {
+            run.job = std::make_unique<prover::Job>(r.p, streams[slot]);
+            run.job->enqueue(r.seed);
+          } catch (const Failure& f) {
+            st = f.status;
+            run.job.reset();
+          }
+        }
+        if (st != STWO_HIP_OK) {
+          std::lock_guard<std::mutex> l(m);
+          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now()});
+          cv_done.notify_all();
+          continue;
+        }
+        slot_busy[slot] = true;
+        inflight_bytes += run.bytes;
+        running.push_back(std::move(run));
+        progressed = true;
+      }
+      if (progressed) continue;
+      std::unique_lock<std::mutex> l(m);
+      if (stopping && queue.empty() && running.empty()) return;
+      if (running.empty())
+        cv_work.wait(l, [&] { return stopping || !queue.empty(); });
+      else
+        cv_work.wait_for(l, std::chrono::microseconds(20));
+    }
+  }
+};
+
+}  // namespace serve
+}  // namespace sh
+
+using namespace sh;
+
+struct StwoProofServer {
+  serve::Server s;
+};
+
+static prover::Params params_of(const StwoWideFibParams* p) {
+  require(p != nullptr, "wide fibonacci: null parameters");
+  prover::Params q{p->n_cols, p->log_n, p->n_queries, p->pow_bits, p->last_log};
+  prover::validate(q);
+  return q;
+}
+
+extern "C" int stwo_hip_wide_fib_proof_size(const StwoWideFibParams* params, uint64_t* proof_words,
+                                            uint64_t* device_bytes) {
+  return guarded([&] {
+    prover::Params p = params_of(params);
+    if (proof_words) *proof_words = prover::Layout(p).total;
+    if (device_bytes) *device_bytes = prover::footprint_bytes(p);
+  });
+}
+
+extern "C" int stwo_hip_wide_fib_prove(const StwoWideFibParams* params, uint64_t seed, uint32_t* proof,
+                                       uint64_t proof_capacity_words) {
+  return guarded([&] {
+    prover::Params p = params_of(params);
+    require(proof != nullptr && proof_capacity_words >= prover::Layout(p).total,
+            "wide fibonacci prove: proof buffer too small");
+    prover::Job job(p, ctx().stream);
+    job.enqueue(seed);
+    hip_check(hipEventSynchronize(job.done), "hipEventSynchronize");
+    sync();
+    const uint32_t* r = job.result();
+    require(!(r[job.L.nonce] == 0xffffffffu && r[job.L.nonce + 1] == 0xffffffffu),
+            "wide fibonacci prove: no proof-of-work nonce within the search bound");
+    std::copy(r, r + job.L.total, proof);
+  });
+}
+
+extern "C" int stwo_hip_server_create(StwoProofServer** out, uint64_t hbm_budget_bytes, uint32_t max_inflight) {
+  return guarded([&] {
+    require(out != nullptr && max_inflight >= 1 && max_inflight <= 16, "server: max_inflight must be in [1, 16]");
+    auto h = std::make_unique<StwoProofServer>();
+    serve::Server& s = h->s;
+    s.budget = hbm_budget_bytes;
+    s.max_inflight = max_inflight;
+    s.streams.resize(max_inflight);
+    s.slot_busy.assign(max_inflight, false);
+    for (auto& st : s.streams) hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    const int dev = ctx().device;
+    s.worker = std::thread([&s, dev] {
+      (void)hipSetDevice(dev);
+      s.loop();
+    });
+    *out = h.release();
+  });
+}
+
+extern "C" int stwo_hip_server_submit(StwoProofServer* h, const StwoWideFibParams* params, uint64_t seed,
+                                      uint64_t* job_id) {
+  try {
+    require(h != nullptr, "server: null handle");
+    prover::Params p = params_of(params);
+    serve::Server& s = h->s;
+    std::lock_guard<std::mutex> l(s.m);
+    require(!s.stopping, "server: stopped");
+    const uint64_t id = s.next_id++;
+    s.queue.push_back(serve::Request{id, p, seed, s.now()});
+    if (job_id) *job_id = id;
+    s.cv_work.notify_one();
+    return STWO_HIP_OK;
+  } catch (const Failure& f) {
+    return f.status;
+  }
+}
+
+extern "C" int stwo_hip_server_wait(StwoProofServer* h, uint32_t timeout_us, uint64_t* job_id, int* job_status,
+                                    uint32_t* proof, uint64_t proof_capacity_words, uint64_t* proof_words,
+                                    double* times) {
+  try {
+    require(h != nullptr, "server: null handle");
+    serve::Server& s = h->s;
+    std::unique_lock<std::mutex> l(s.m);
+    if (!s.cv_done.wait_for(l, std::chrono::microseconds(timeout_us), [&] { return !s.results.empty(); }))
+      return STWO_HIP_NOT_READY;
+    serve::Result r = std::move(s.results.front());
+    s.results.pop_front();
+    l.unlock();
+    if (job_id) *job_id = r.id;
+    if (job_status) *job_status = r.status;
+    if (proof_words) *proof_words = r.proof.size();
+    if (proof && proof_capacity_words >= r.proof.size()) std::copy(r.proof.begin(), r.proof.end(), proof);
+    if (times) {
+      times[0] = r.t_submit;
+      times[1] = r.t_start;
+      times[2] = r.t_end;
+    }
+    return STWO_HIP_OK;
+  } catch (const Failure& f) {
+    return f.status;
+  }
+}
+
+extern "C" int stwo_hip_server_destroy(StwoProofServer* h) {
+  if (!h) return STWO_HIP_OK;
+  {
+    std::lock_guard<std::mutex> l(h->s.m);
+    h->s.stopping = true;
+    h->s.cv_work.notify_all();
+  }
+  if (h->s.worker.joinable()) h->s.worker.join();
+  return guarded([&] {
+    for (auto& st : h->s.streams) (void)hipStreamDestroy(st);
+    delete h;
+  });
+}

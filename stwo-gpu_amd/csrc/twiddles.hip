@@ -84,6 +84,8 @@ extern "C" int stwo_hip_twiddles(uint32_t initial_index, uint32_t step_size, uin
       e.itw = (uint32_t*)ctx().pool.alloc(size * 4);
       tw::compute_tree(e.tw, point_at_index(key.initial), point_at_index(key.step), log_size);
       batch_inverse_m31(e.tw, e.itw, size);
+      // the cached tree is shared by work on every stream: finish it before handing it out
+      hip_check(hipStreamSynchronize(ctx().cur), "hipStreamSynchronize");
       it = cache.emplace(key, e).first;
     }
     *twiddles = it->second.tw;

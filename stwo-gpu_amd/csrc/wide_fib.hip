@@ -51,7 +51,65 @@ __global__ __launch_bounds__(256) void quotients_kernel(uint32_t n_rows, uint32_
   q3[r] = mul(reduce63(s3), d);
 }
 
+// Trace of a wide-Fibonacci instance (the AIR of examples/wide_fibonacci.rs): row r has
+// col[0] = U(seed, r), col[1] = U(seed, n + r) (uniform_m31) and
+// col[i] = col[i-2]^2 + col[i-1]^2.  One lane per row; every column store is coalesced.
+__global__ __launch_bounds__(256) void trace_kernel(uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                    uint32_t log_n, const uint64_t* __restrict__ seed_dev) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = 1u << log_n;
+  if (r >= n) return;
+  const uint64_t seed = *seed_dev;
+  uint32_t a = uniform_m31(seed, r), b = uniform_m31(seed, (uint64_t)n + r);
+  cols[0][r] = a;
+  cols[1][r] = b;
+  for (uint32_t i = 2; i < n_cols; i++) {
+    const uint32_t c = add(sqr(a), sqr(b));
+    cols[i][r] = c;
+    a = b;
+    b = c;
+  }
+}
+
+// 1 / Z(p) on the rows of CanonicCoset(log_ext) (bit-reversed order), where
+// Z(p) = pi^(log_n - 1)(p.x), pi(x) = 2x^2 - 1, vanishes exactly on CanonicCoset(log_n).
+__global__ __launch_bounds__(256) void vanishing_inv_kernel(uint32_t* __restrict__ out, uint32_t log_ext,
+                                                            uint32_t log_n) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (1u << log_ext)) return;
+  const uint32_t hi = 1u << (30 - log_ext), hs = (1u << (32 - log_ext)) & 0x7fffffffu;
+  uint32_t x = point_at_index(circle_domain_index(hi, hs, log_ext - 1, brev(r, log_ext))).x;
+  for (uint32_t i = 1; i < log_n; i++) x = sub(add(sqr(x), sqr(x)), 1);
+  out[r] = inv(x);
+}
+
+// powers[k] = alpha^k (QM31 AoS), alpha read from device memory
+__global__ __launch_bounds__(256) void powers_kernel(const qm31* __restrict__ alpha, uint32_t n, uint4* powers) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const qm31 v = qpow(*alpha, k);
+  powers[k] = make_uint4(v.x.a, v.x.b, v.y.a, v.y.b);
+}
+
 }  // namespace wfib
+
+void wide_fib_trace_async(uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_n, const uint64_t* d_seed) {
+  require(n_cols >= 2 && log_n >= 1 && log_n <= 30, "wide fibonacci trace: bad sizes");
+  timed("wide_fib_trace", [&] {
+    wfib::trace_kernel<<<ceil_div(1ull << log_n, 256), 256, 0, ctx().cur>>>(d_cols, n_cols, log_n, d_seed);
+  });
+}
+void vanishing_inverse_async(uint32_t* d_out, uint32_t log_ext, uint32_t log_n) {
+  require(log_n >= 1 && log_ext > log_n && log_ext <= 30, "vanishing inverse: bad sizes");
+  timed("vanishing_inverse", [&] {
+    wfib::vanishing_inv_kernel<<<ceil_div(1ull << log_ext, 256), 256, 0, ctx().cur>>>(d_out, log_ext, log_n);
+  });
+}
+void qm31_powers_async(const qm31* d_alpha, uint32_t n, uint32_t* d_powers) {
+  timed("qm31_powers", [&] {
+    wfib::powers_kernel<<<ceil_div(n, 256), 256, 0, ctx().cur>>>(d_alpha, n, (uint4*)d_powers);
+  });
+}
 
 void wide_fib_quotients_async(uint32_t* const q[4], const uint32_t* const* d_trace, const uint32_t* powers,
                               const uint32_t* dinv, uint32_t n_rows, uint32_t n_cols) {

@@ -25,6 +25,10 @@ u32p = C.POINTER(C.c_uint32)
 vp = C.c_void_p
 
 
+class WideFibParams(C.Structure):
+    _fields_ = [("n_cols", u32), ("log_n", u32), ("n_queries", u32), ("pow_bits", u32), ("last_log", u32)]
+
+
 class QM31(C.Structure):
     _fields_ = [("a", u32), ("b", u32), ("c", u32), ("d", u32)]
 
@@ -118,6 +122,14 @@ def _declare(L):
                                                   C.c_int]),
         "stwo_hip_commit_plan_run": (C.c_int, [vp, C.c_int]),
         "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
+        "stwo_hip_grind": (C.c_int, [u32p, u32, C.POINTER(u64)]),
+        "stwo_hip_wide_fib_proof_size": (C.c_int, [C.POINTER(WideFibParams), C.POINTER(u64), C.POINTER(u64)]),
+        "stwo_hip_wide_fib_prove": (C.c_int, [C.POINTER(WideFibParams), u64, u32p, u64]),
+        "stwo_hip_server_create": (C.c_int, [C.POINTER(vp), u64, u32]),
+        "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
+        "stwo_hip_server_wait": (C.c_int, [vp, u32, C.POINTER(u64), C.POINTER(C.c_int), u32p, u64,
+                                           C.POINTER(u64), C.POINTER(C.c_double)]),
+        "stwo_hip_server_destroy": (C.c_int, [vp]),
         "stwo_hip_timer_begin": (C.c_int, []),
         "stwo_hip_timer_end": (C.c_int, [C.POINTER(C.c_float)]),
         "stwo_hip_kernel_timing": (C.c_int, [C.c_int]),
@@ -664,6 +676,71 @@ class CommitPlan:
     def close(self):
         if self.handle:
             lib().stwo_hip_commit_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- proofs
+NOT_READY = 5
+
+
+def grind(digest, pow_bits):
+    d = np.ascontiguousarray(digest, np.uint32)
+    out = u64()
+    _c(lib().stwo_hip_grind(d.ctypes.data_as(u32p), pow_bits, C.byref(out)))
+    return out.value
+
+
+def wide_fib_params(n_cols, log_n, n_queries=3, pow_bits=10, last_log=2):
+    return WideFibParams(n_cols, log_n, n_queries, pow_bits, last_log)
+
+
+def wide_fib_proof_size(params):
+    w, b = u64(), u64()
+    _c(lib().stwo_hip_wide_fib_proof_size(C.byref(params), C.byref(w), C.byref(b)))
+    return w.value, b.value
+
+
+def wide_fib_prove(params, seed):
+    words, _ = wide_fib_proof_size(params)
+    out = np.zeros(words, np.uint32)
+    _c(lib().stwo_hip_wide_fib_prove(C.byref(params), seed, out.ctypes.data_as(u32p), words))
+    return out
+
+
+class ProofServer:
+    """Continuous-batching proof server (stwo_hip_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, max_inflight=4):
+        h = vp()
+        _c(lib().stwo_hip_server_create(C.byref(h), hbm_budget_bytes, max_inflight))
+        self.handle = h.value
+
+    def submit(self, params, seed):
+        jid = u64()
+        _c(lib().stwo_hip_server_submit(self.handle, C.byref(params), seed, C.byref(jid)))
+        return jid.value
+
+    def wait(self, timeout_us=1000, capacity_words=1 << 22):
+        """(job_id, status, proof, (t_submit, t_start, t_end)) or None after the timeout."""
+        jid, st, nw = u64(), C.c_int(), u64()
+        buf = np.zeros(capacity_words, np.uint32)
+        times = (C.c_double * 3)()
+        r = lib().stwo_hip_server_wait(self.handle, timeout_us, C.byref(jid), C.byref(st), buf.ctypes.data_as(u32p),
+                                       capacity_words, C.byref(nw), times)
+        if r == NOT_READY:
+            return None
+        _c(r)
+        return jid.value, st.value, buf[:nw.value].copy(), tuple(times)
+
+    def close(self):
+        if self.handle:
+            _c(lib().stwo_hip_server_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
