@@ -81,12 +81,18 @@ def bench_lde_commit(args):
     root = S.canonic_half_coset(LOG_N + LOG_BLOWUP)
     tw, itw = S.cached_twiddles(root)
     tree_len = 1 << root.log_size
-    tree = S.MerkleTree(ldes, [LOG_N + LOG_BLOWUP] * C)
+    # one plan: pointer tables uploaded once, FFT pass schedule autotuned on these buffers,
+    # every step replayed as one hipGraph (kernel-timing passes run it unrecorded)
+    plan = S.CommitPlan(evals, coeffs, ldes, [LOG_N] * C, LOG_BLOWUP, tw, itw, tree_len, autotune=True)
+    tree = plan
+    schedule = S.lde_schedule(LOG_N, LOG_BLOWUP, C)
     S.synchronize()
 
     def step():
-        S.lde_columns_async(evals, coeffs, ldes, [LOG_N] * C, LOG_BLOWUP, tw, itw, tree_len)
-        tree.commit_async()
+        plan.run(True)
+
+    def step_eager():
+        plan.run(False)
 
     for _ in range(args.warmup):
         step()
@@ -107,7 +113,7 @@ def bench_lde_commit(args):
     S.kernel_timing(True)
     reps = 3
     for _ in range(reps):
-        step()
+        step_eager()
     S.synchronize()
     S.kernel_timing(False)
     names = ["fft_inv_contig", "fft_inv_strided", "fft_lde_strided", "fft_fwd_strided", "fft_fwd_contig", "merkle_leaf",
@@ -126,12 +132,7 @@ def bench_lde_commit(args):
     achieved = alg[dom] / (d_avg * 1e-3) / 1e9
     pipe = pipeline_bytes(C, n)
     root_hash = tree.root()
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "lde_merkle_256x2e20_blowup2_traffic.json")
-    if os.path.exists(tf):  # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)
-        rec = json.load(open(tf))["kernels"].get(dom)
-        if rec:
-            traffic = rec["read_bytes"] + rec["write_bytes"]
+    traffic = _traffic("lde_merkle_256x2e20_blowup2", dom)
 
     out = {
         "metric": "LDE+Merkle M31 elems/s",
@@ -147,7 +148,9 @@ def bench_lde_commit(args):
         "dtype": "u32 (M31)",
         "data": "synthetic (uniform M31, seeded counter-based stream)",
         "config": {"workload": "cfg2 LDE+commit: 256 M31 columns log 20 -> LDE log 21 (blowup 2) + Blake2s Merkle",
-                   "columns": C, "log_size": LOG_N, "log_blowup": LOG_BLOWUP, "parallelism": "single GPU"},
+                   "columns": C, "log_size": LOG_N, "log_blowup": LOG_BLOWUP, "parallelism": "single GPU",
+                   "fft_schedule": {"contiguous_layers": schedule[0], "fused_strided": schedule[1]},
+                   "launch": "hipGraph replay of the commit plan"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg[dom], "avg_launch_ms": d_avg},
@@ -205,11 +208,21 @@ def _line(metric, value, unit, ms, args, config, roofline=None, extra=None, high
     return out
 
 
-def _roof(bytes_, avg_ms, kernel):
+def _traffic(profile, kernel):
+    """Measured HBM bytes per dispatch of `kernel` (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration) from profiles/<profile>_traffic.json, or None."""
+    tf = os.path.join(ROOT, "profiles", profile + "_traffic.json")
+    if not os.path.exists(tf):
+        return None
+    rec = json.load(open(tf))["kernels"].get(kernel)
+    return rec["read_bytes"] + rec["write_bytes"] if rec else None
+
+
+def _roof(bytes_, avg_ms, kernel, profile=None):
     gbs = bytes_ / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes_per_launch": bytes_,
-            "avg_launch_ms": avg_ms}
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": _traffic(profile, kernel) if profile else None,
+            "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms}
 
 
 def bench_cfg1(args):
@@ -263,7 +276,7 @@ def bench_batch_inverse(args):
     ks = _kernel_avg(S, step, ["batch_inverse_qm31"])
     avg = ks["batch_inverse_qm31"]["avg_ms"]
     return _line("batch_inverse QM31 elems/s", n / (ms * 1e-3), "elems/s", ms, args,
-                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"}, _roof(32 * n, avg, "batch_inverse_qm31"),
+                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"}, _roof(32 * n, avg, "batch_inverse_qm31", "batch_inverse_qm31_2e24"),
                  {"kernels": ks, "note": "ms_per_step is the synchronous ABI call (launch + sync)"})
 
 
@@ -294,7 +307,7 @@ def bench_fri(args):
     total = circle_bytes + line_bytes
     return _line("FRI fold chain GB/s (log 22 circle -> line -> log 4)", total / (ms * 1e-3) / 1e9, "GB/s", ms, args,
                  {"workload": "cfg2 FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
-                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line"),
+                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line", "fri_fold_2e22_to_2e4"),
                  {"kernels": ks, "algorithmic_bytes_per_step": total})
 
 
@@ -329,7 +342,7 @@ def bench_quotients(args):
     n_samples = sum(len(cv) for _, cv in batches)
     return _line("accumulate_quotients rows/s", n / (ms * 1e-3), "rows/s", ms, args,
                  {"workload": f"cfg3 quotients: {C} M31 columns log {L}, 3 QM31 points, {n_samples} samples"},
-                 _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients"), {"kernels": ks})
+                 _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients", "quotients_200x2e21_3pts"), {"kernels": ks})
 
 
 def cfg4_log_sizes(n_cols=1200, seed=0x5EED0004):
@@ -451,8 +464,92 @@ def bench_batched(args):
                   "algorithmic_bytes_per_launch": pipe, "avg_launch_ms": round(ms_graph, 4)}, extra)
 
 
+def serving_jobs(n_jobs, rate, seed=0x5EED0005):
+    """Poisson arrivals (seeded): (arrival_s, n_cols, log_n, trace seed)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    cols = rng.choice([32, 64, 128, 256, 512], n_jobs)
+    logs = rng.integers(12, 21, n_jobs)
+    return [(float(t[i]), int(cols[i]), int(logs[i]), 0xA000 + i) for i in range(n_jobs)]
+
+
+def _serve(S, jobs, budget, slots, pow_bits, last_log):
+    """Submit each job at its arrival time; returns per-job ((submit, start, end), proof words)."""
+    srv = S.ProofServer(budget, slots)
+    t0 = time.perf_counter()
+    done, ids = {}, {}
+    i = 0
+    while len(done) < len(jobs):
+        now = time.perf_counter() - t0
+        while i < len(jobs) and jobs[i][0] <= now:
+            _, C, ln, seed = jobs[i]
+            ids[srv.submit(S.wide_fib_params(C, ln, 3, pow_bits, last_log), seed)] = i
+            i += 1
+        wait_us = 200 if i < len(jobs) else 100_000
+        if i < len(jobs):
+            wait_us = max(20, min(wait_us, int((jobs[i][0] - now) * 1e6)))
+        r = srv.wait(wait_us)
+        while r is not None:
+            jid, st, proof, times = r
+            if st != 0:
+                raise RuntimeError(f"job {ids[jid]} failed with status {st}")
+            done[ids[jid]] = (times, proof.size)
+            r = srv.wait(0)
+    srv.close()
+    return done
+
+
+def bench_serving(args):
+    """Wide-Fibonacci proof serving: a seeded Poisson stream of prove jobs (n_cols in
+    {32..512}, log_n in {12..20}, blowup 2, 3 queries) through the continuous-batching
+    server, against the same stream proved one job at a time (plain per-job path)."""
+    import numpy as np
+    import stwo_hip as S
+
+    n_jobs, pow_bits, last_log = args.serve_jobs, 10, 2
+    budget = args.serve_budget_gib << 30
+    rate = args.serve_rate
+    jobs = serving_jobs(n_jobs, rate)
+    # warm the twiddle cache and the memory pool with one pass of every shape, off the clock
+    shapes = sorted({(C, ln) for _, C, ln, _ in jobs})
+    warm = S.ProofServer(budget, args.serve_slots)
+    for C, ln in shapes:
+        warm.submit(S.wide_fib_params(C, ln, 3, pow_bits, last_log), 1)
+    for _ in shapes:
+        assert warm.wait(60_000_000) is not None
+    warm.close()
+
+    def summarize(done, n):
+        lat = np.array([done[i][0][2] - done[i][0][0] for i in range(n)])
+        first = min(done[i][0][0] for i in range(n))
+        last = max(done[i][0][2] for i in range(n))
+        return {"proofs_per_s": round(n / (last - first), 1), "p50_ms": round(float(np.percentile(lat, 50) * 1e3), 2),
+                "p99_ms": round(float(np.percentile(lat, 99) * 1e3), 2), "makespan_s": round(last - first, 4),
+                "host_ms_per_job": round(float(np.mean([done[i][0][3] for i in range(n)])) * 1e3, 3)}
+
+    batched = summarize(_serve(S, jobs, budget, args.serve_slots, pow_bits, last_log), n_jobs)
+    plain = summarize(_serve(S, jobs, budget, 1, pow_bits, last_log), n_jobs)
+    sweep = []
+    for r in (args.serve_sweep or []):
+        js = serving_jobs(n_jobs, r)
+        sweep.append({"rate": r, "batched": summarize(_serve(S, js, budget, args.serve_slots, pow_bits, last_log), n_jobs),
+                      "plain": summarize(_serve(S, js, budget, 1, pow_bits, last_log), n_jobs)})
+    elems = sum(C << ln for _, C, ln, _ in jobs)
+    extra = {"batched": batched, "plain_one_at_a_time": plain,
+             "throughput_ratio": batched["proofs_per_s"] / plain["proofs_per_s"],
+             "jobs": n_jobs, "arrival_rate_per_s": rate, "slots": args.serve_slots,
+             "hbm_budget_gib": args.serve_budget_gib, "trace_elements_total": elems,
+             "pow_bits": pow_bits, "last_layer_log": last_log, "rate_sweep": sweep}
+    return _line("served wide-Fibonacci proofs/s (continuous batching)", batched["proofs_per_s"], "proofs/s",
+                 1e3 / batched["proofs_per_s"], args,
+                 {"workload": f"cfg5 serving: {n_jobs} Poisson(rate {rate}/s) wide-Fibonacci prove jobs, n_cols in "
+                              "{32,64,128,256,512}, log_n in 12..20, blowup 2, 3 queries, Blake2s channel"},
+                 extra=extra)
+
+
 CONFIGS = {"cfg2": bench_lde_commit, "cfg1": bench_cfg1, "cfg2-binv": bench_batch_inverse, "cfg2-fri": bench_fri,
-           "cfg3": bench_quotients, "cfg4": bench_batched}
+           "cfg3": bench_quotients, "cfg4": bench_batched, "cfg5": bench_serving}
 
 
 def main():
@@ -463,6 +560,11 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-sample-cols", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate (jobs/s)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gib", type=int, default=96)
+    ap.add_argument("--serve-sweep", type=float, nargs="*", help="extra arrival rates to run both modes at")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()

@@ -260,6 +260,10 @@ int stwo_hip_commit_plan_create(StwoCommitPlan **plan, uint32_t n_cols, const ui
                                 uint64_t tree_len, StwoBlake2sHash *const *layers, uint32_t max_log, int autotune);
 int stwo_hip_commit_plan_run(StwoCommitPlan *plan, int use_graph);
 int stwo_hip_commit_plan_destroy(StwoCommitPlan *plan);
+/* The FFT pass schedule in use for an LDE size class (autotuned or default): layers per
+ * contiguous pass (12 or 13) and whether the strided passes are fused. */
+int stwo_hip_lde_schedule(uint32_t log_size, uint32_t log_blowup, uint64_t n_cols, uint32_t *contiguous_layers,
+                          int *fused);
 
 /* Proof of work (GrindOps<Blake2sChannel>::grind, which the reference delegates to the CPU
  * SimdBackend, stwo_gpu_backend/src/backend.rs:14-18): the smallest nonce whose
@@ -286,8 +290,9 @@ int stwo_hip_wide_fib_prove(const StwoWideFibParams *params, uint64_t seed, uint
 /* Proof server with continuous batching: jobs are admitted in arrival order while fewer than
  * max_inflight run and their device footprints fit hbm_budget_bytes; each runs on its own
  * stream and overlaps with the others.  wait() returns STWO_HIP_NOT_READY after timeout_us
- * without a finished job; otherwise the job's id, status, proof (when it fits) and its
- * submit / start / end times in seconds since the server was created. */
+ * without a finished job; otherwise the job's id, status, proof (when it fits) and in
+ * times[0..3] its submit / start / end times in seconds since the server was created and
+ * the host time spent building and enqueuing it. */
 typedef struct StwoProofServer StwoProofServer;
 int stwo_hip_server_create(StwoProofServer **server, uint64_t hbm_budget_bytes, uint32_t max_inflight);
 int stwo_hip_server_submit(StwoProofServer *server, const StwoWideFibParams *params, uint64_t seed,

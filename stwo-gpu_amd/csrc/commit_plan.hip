@@ -213,6 +213,15 @@ extern "C" int stwo_hip_commit_plan_run(StwoCommitPlan* plan, int use_graph) {
   });
 }
 
+extern "C" int stwo_hip_lde_schedule(uint32_t log_size, uint32_t log_blowup, uint64_t n_cols, uint32_t* contiguous_layers,
+                                     int* fused) {
+  return guarded([&] {
+    const fft::LdeVariant v = fft::lde_variant_for(log_size, log_blowup, n_cols);
+    if (contiguous_layers) *contiguous_layers = v.kc;
+    if (fused) *fused = v.fused ? 1 : 0;
+  });
+}
+
 extern "C" int stwo_hip_commit_plan_destroy(StwoCommitPlan* plan) {
   return guarded([&] {
     if (!plan) return;

@@ -15,6 +15,7 @@
 // a row costs one QM31 x M31 multiply-accumulate per sample, done on 64-bit
 // lazy sums (v_mad_u64_u32) and reduced once per batch.  Sample tables are
 // uniform across the wavefront and are read with scalar loads.
+#include <map>
 #include <vector>
 
 #include "field.hpp"
@@ -81,6 +82,77 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
     cm31 d = csub(cmul(cm31{sub(B.prx.a, p.x), B.prx.b}, B.piy), cmul(cm31{sub(B.pry.a, p.y), B.pry.b}, B.pix));
     acc = qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
   }
+  o0[row] = acc.x.a;
+  o1[row] = acc.x.b;
+  o2[row] = acc.y.a;
+  o3[row] = acc.y.b;
+}
+
+// Column-major form for up to 4 batches: each column value of a row is loaded once and
+// added into the sums of every batch that samples the column (the batch-major kernel
+// above loads a column once per batch that samples it).  Columns are visited in groups
+// of four whose loads are issued together.  Host-planned folds keep each 64-bit sum
+// below 2^64 (at most three products between folds).
+struct ColHead {
+  uint32_t col, first, count, pad;  // entries [first, first + count) sample column col
+};
+struct ColEntry {
+  uint32_t batch, fold;  // fold: reduce this batch's sums after adding this product
+  qm31 alpha;
+};
+
+__device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)[4], pt p, qm31 acc) {
+  qm31 sum = {{reduce63(s[0]), reduce63(s[1])}, {reduce63(s[2]), reduce63(s[3])}};
+  qm31 num = qsub(qmul(B.c, sum), qadd(qscale(B.a_sum, p.y), B.b_sum));
+  cm31 d = csub(cmul(cm31{sub(B.prx.a, p.x), B.prx.b}, B.piy), cmul(cm31{sub(B.pry.a, p.y), B.pry.b}, B.pix));
+  return qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
+                                                              const uint32_t* const* __restrict__ cols,
+                                                              const ColHead* __restrict__ heads, uint32_t n_heads,
+                                                              const ColEntry* __restrict__ ents,
+                                                              const Batch* __restrict__ batches,
+                                                              uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
+                                                              uint32_t* __restrict__ o2, uint32_t* __restrict__ o3) {
+  uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (1u << log_size)) return;
+  uint64_t s[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; b++) s[b][0] = s[b][1] = s[b][2] = s[b][3] = 0;
+  for (uint32_t h = 0; h < n_heads; h += 4) {
+    uint32_t v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = h + i < n_heads ? cols[heads[h + i].col][row] : 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (h + i >= n_heads) break;
+      const ColHead H = heads[h + i];
+      for (uint32_t e = H.first; e < H.first + H.count; e++) {
+        const ColEntry E = ents[e];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          if (E.batch == (uint32_t)b) {
+            s[b][0] += (uint64_t)E.alpha.x.a * v[i];
+            s[b][1] += (uint64_t)E.alpha.x.b * v[i];
+            s[b][2] += (uint64_t)E.alpha.y.a * v[i];
+            s[b][3] += (uint64_t)E.alpha.y.b * v[i];
+            if (E.fold) {
+              s[b][0] = fold64(s[b][0]);
+              s[b][1] = fold64(s[b][1]);
+              s[b][2] = fold64(s[b][2]);
+              s[b][3] = fold64(s[b][3]);
+            }
+          }
+        }
+      }
+    }
+  }
+  pt p = point_at_index(circle_domain_index(hi, hs, log_size - 1, brev(row, log_size)));
+  qm31 acc = qzero();
+#pragma unroll
+  for (int b = 0; b < NB; b++) acc = finish_batch(batches[b], s[b], p, acc);
   o0[row] = acc.x.a;
   o1[row] = acc.x.b;
   o2[row] = acc.y.a;
@@ -181,6 +253,55 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
       ss.push_back(quot::Sample{col_idx[off + j], 0, alpha});
     }
     off += sizes[b];
+  }
+  if (n_batches >= 1 && n_batches <= 4) {
+    // column-major program: every sampled column once, with its (batch, alpha) entries
+    std::map<uint32_t, std::vector<std::pair<uint32_t, qm31>>> by_col;
+    for (uint32_t b = 0; b < n_batches; b++)
+      for (uint32_t j = 0; j < bs[b].count; j++) {
+        const quot::Sample& S = ss[bs[b].first + j];
+        by_col[S.col].push_back({b, S.alpha});
+      }
+    std::vector<quot::ColHead> heads;
+    std::vector<quot::ColEntry> ents;
+    uint32_t pending[4] = {0, 0, 0, 0};
+    for (auto& kv : by_col) {
+      heads.push_back(quot::ColHead{kv.first, (uint32_t)ents.size(), (uint32_t)kv.second.size(), 0});
+      for (auto& e : kv.second) {
+        const bool fold = ++pending[e.first] == 3;
+        if (fold) pending[e.first] = 0;
+        ents.push_back(quot::ColEntry{e.first, fold ? 1u : 0u, e.second});
+      }
+    }
+    DevBuf dh = upload(heads.data(), heads.size() * sizeof(quot::ColHead));
+    DevBuf de = upload(ents.data(), ents.size() * sizeof(quot::ColEntry));
+    DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
+    timed("accumulate_quotients", [&] {
+      const uint32_t blocks = ceil_div(domain_size, 256);
+      const uint32_t nh = (uint32_t)heads.size();
+      auto* h = dh.as<quot::ColHead>();
+      auto* e = de.as<quot::ColEntry>();
+      auto* b = dbs.as<quot::Batch>();
+      hipStream_t st = ctx().cur;
+      switch (n_batches) {
+        case 1:
+          quot::accumulate_cols_kernel<1><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
+                                                                    out[1], out[2], out[3]);
+          break;
+        case 2:
+          quot::accumulate_cols_kernel<2><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
+                                                                    out[1], out[2], out[3]);
+          break;
+        case 3:
+          quot::accumulate_cols_kernel<3><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
+                                                                    out[1], out[2], out[3]);
+          break;
+        default:
+          quot::accumulate_cols_kernel<4><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
+                                                                    out[1], out[2], out[3]);
+      }
+    });
+    return;
   }
   if (ss.empty()) ss.push_back(quot::Sample{0, 0, qzero()});
   if (bs.empty()) bs.push_back(quot::Batch{});

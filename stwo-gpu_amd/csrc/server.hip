@@ -38,13 +38,13 @@ struct Running {
   std::unique_ptr<prover::Job> job;
   int slot;
   uint64_t bytes;
-  double t_start;
+  double t_start, host_s;  // admission time, host time spent building and enqueuing the job
 };
 struct Result {
   uint64_t id;
   int status;
   std::vector<uint32_t> proof;
-  double t_submit, t_start, t_end;
+  double t_submit, t_start, t_end, host_s;
 };
 
 struct Server {
@@ -68,7 +68,7 @@ struct Server {
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
   void finish(Running& r, int status) {
-    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now()};
+    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now(), r.host_s};
     if (status == STWO_HIP_OK) {
       const uint32_t* pr = r.job->result();
       res.proof.assign(pr, pr + r.job->L.total);
@@ -113,13 +113,14 @@ struct Server {
         }
         int slot = 0;
         while (slot_busy[slot]) slot++;
-        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), now()};
+        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), now(), 0.0};
         int st = STWO_HIP_OK;
         {
           auto g = lock();
           try {
             run.job = std::make_unique<prover::Job>(r.p, streams[slot]);
             run.job->enqueue(r.seed);
+            run.host_s = now() - run.t_start;
           } catch (const Failure& f) {
             st = f.status;
             run.job.reset();
@@ -127,7 +128,7 @@ struct Server {
         }
         if (st != STWO_HIP_OK) {
           std::lock_guard<std::mutex> l(m);
-          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now()});
+          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now(), 0.0});
           cv_done.notify_all();
           continue;
         }
@@ -246,6 +247,7 @@ extern "C" int stwo_hip_server_wait(StwoProofServer* h, uint32_t timeout_us, uin
       times[0] = r.t_submit;
       times[1] = r.t_start;
       times[2] = r.t_end;
+      times[3] = r.host_s;
     }
     return STWO_HIP_OK;
   } catch (const Failure& f) {

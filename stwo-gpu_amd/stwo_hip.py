@@ -122,6 +122,7 @@ def _declare(L):
                                                   C.c_int]),
         "stwo_hip_commit_plan_run": (C.c_int, [vp, C.c_int]),
         "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
+        "stwo_hip_lde_schedule": (C.c_int, [u32, u32, u64, u32p, C.POINTER(C.c_int)]),
         "stwo_hip_grind": (C.c_int, [u32p, u32, C.POINTER(u64)]),
         "stwo_hip_wide_fib_proof_size": (C.c_int, [C.POINTER(WideFibParams), C.POINTER(u64), C.POINTER(u64)]),
         "stwo_hip_wide_fib_prove": (C.c_int, [C.POINTER(WideFibParams), u64, u32p, u64]),
@@ -649,6 +650,13 @@ def random_circle_point_qm31(rng):
         return x, y
 
 
+def lde_schedule(log_size, log_blowup, n_cols):
+    """(layers per contiguous pass, fused strided passes) in use for this LDE size class."""
+    kc, fused = C.c_uint32(), C.c_int()
+    _c(lib().stwo_hip_lde_schedule(log_size, log_blowup, n_cols, C.byref(kc), C.byref(fused)))
+    return kc.value, bool(fused.value)
+
+
 class CommitPlan:
     """Batched LDE + Merkle commitment of columns of mixed sizes (stwo_hip_commit_plan_*)."""
 
@@ -726,13 +734,15 @@ class ProofServer:
         _c(lib().stwo_hip_server_submit(self.handle, C.byref(params), seed, C.byref(jid)))
         return jid.value
 
-    def wait(self, timeout_us=1000, capacity_words=1 << 22):
-        """(job_id, status, proof, (t_submit, t_start, t_end)) or None after the timeout."""
+    def wait(self, timeout_us=1000):
+        """(job_id, status, proof, (t_submit, t_start, t_end, host_s)) or None after the timeout."""
         jid, st, nw = u64(), C.c_int(), u64()
-        buf = np.zeros(capacity_words, np.uint32)
-        times = (C.c_double * 3)()
+        if getattr(self, "_buf", None) is None:
+            self._buf = np.zeros(1 << 20, np.uint32)
+        buf = self._buf
+        times = (C.c_double * 4)()
         r = lib().stwo_hip_server_wait(self.handle, timeout_us, C.byref(jid), C.byref(st), buf.ctypes.data_as(u32p),
-                                       capacity_words, C.byref(nw), times)
+                                       buf.size, C.byref(nw), times)
         if r == NOT_READY:
             return None
         _c(r)

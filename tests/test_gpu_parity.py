@@ -346,14 +346,16 @@ def test_accumulate_quotients_reference_shape():
     assert (got.to_numpy() == want).all()
 
 
-def test_accumulate_quotients_uneven_batches():
-    # batches of different sizes: the reference kernel's line-coefficient offset bug case
-    rng = random.Random(7)
+@pytest.mark.parametrize("sizes", [[5, 1, 9, 2], [7], [3, 16], [20, 20, 20], [4, 4, 4, 4, 4], [1, 1, 1, 1, 1, 1]])
+def test_accumulate_quotients_uneven_batches(sizes):
+    # batches of different sizes: the reference kernel's line-coefficient offset bug case;
+    # up to 4 batches take the column-major kernel, more take the batch-major one
+    rng = random.Random(7 + len(sizes))
     m, n_cols = 12, 20
     root = S.canonic_half_coset(m)
     cols = [rand_m31(1 << m, 300 + i) for i in range(n_cols)]
     batches = []
-    for b, size in enumerate([5, 1, 9, 2]):
+    for b, size in enumerate(sizes):
         idx = rng.sample(range(n_cols), size)
         batches.append((_point(rng), [(c, rand_qm31(rng)) for c in idx]))
     rc = rand_qm31(rng)
