@@ -275,6 +275,15 @@ int stwo_hip_grind(const uint32_t *digest, uint32_t pow_bits, uint64_t *nonce);
  * proved end to end on the device: trace LDE + commitment, composition, OODS samples, DEEP
  * quotients, FRI, proof of work, decommitments).  The trace is generated on the device from
  * `seed`.  The proof is a word array whose layout the header (first 8 words) describes. */
+/* The trace of the wide-Fibonacci statement on the library stream: column i of 2^log_n
+ * rows, col[0][r] = U(seed, r), col[1][r] = U(seed, 2^log_n + r) (the stream of
+ * stwo_hip_fill_uniform_m31), col[i] = col[i-2]^2 + col[i-1]^2.  columns: HOST array. */
+int stwo_hip_wide_fib_trace(uint32_t *const *columns, uint32_t n_cols, uint32_t log_n, uint64_t seed);
+/* out[r] = 1 / Z(p_r) on the rows of CanonicCoset(log_ext) (bit-reversed), where
+ * Z = pi^(log_n - 1)(x) vanishes on CanonicCoset(log_n) (the denominators of
+ * examples/wide_fibonacci.rs:101-104, computed on the device). */
+int stwo_hip_vanishing_inverse(uint32_t *out, uint32_t log_ext, uint32_t log_n);
+
 typedef struct {
   uint32_t n_cols;    /* >= 3 */
   uint32_t log_n;     /* trace rows = 2^log_n, 4 .. 26 */

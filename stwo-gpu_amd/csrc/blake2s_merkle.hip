@@ -13,6 +13,7 @@
 // built by a subtree kernel that produces up to 11 layers per launch in LDS,
 // injecting the columns of any layer (mixed-degree commitments) as it goes.
 #include <algorithm>
+#include <cstring>
 
 #include "field.hpp"
 #include "merkle_plan.hpp"
@@ -172,10 +173,32 @@ static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 // run in parallel.  Runs of layers with few columns are built by subtree launches.
 constexpr uint32_t HEAVY_CHUNKS = 2;
 
+size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log) {
+  const size_t cols = ((size_t)std::max<uint32_t>(n_cols, 1) * sizeof(void*) + 15) & ~(size_t)15;
+  const size_t lcs = ((size_t)(max_log + 1) * sizeof(LayerCols) + 15) & ~(size_t)15;
+  return cols + lcs + (size_t)(max_log + 1) * sizeof(Hash*);
+}
+
 MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
                        uint32_t max_log_)
     : max_log(max_log_) {
   require(max_log <= 31, "merkle: log size too large");
+  const size_t bytes = table_bytes(n_cols, max_log);
+  own = DevBuf(bytes);
+  std::vector<char> host(bytes);
+  build(n_cols, cols, logs, layers, own.as<char>(), host.data());
+  upload_to(own.p, host.data(), bytes);
+}
+
+MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+                       uint32_t max_log_, void* dev, void* host_image)
+    : max_log(max_log_) {
+  require(max_log <= 31, "merkle: log size too large");
+  build(n_cols, cols, logs, layers, (char*)dev, (char*)host_image);
+}
+
+void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+                       char* dev, char* host) {
   // stable sort by decreasing size (MerkleProver::commit)
   std::vector<std::vector<const uint32_t*>> by_layer(max_log + 1);
   for (uint32_t i = 0; i < n_cols; i++) {
@@ -189,13 +212,17 @@ MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint3
     flat.insert(flat.end(), by_layer[k].begin(), by_layer[k].end());
   }
   if (flat.empty()) flat.push_back(nullptr);
-  dcols = upload(flat.data(), flat.size() * sizeof(void*));
+  const size_t o_lcs = (flat.size() * sizeof(void*) + 15) & ~(size_t)15;
+  const size_t o_outs = o_lcs + (((size_t)(max_log + 1) * sizeof(LayerCols) + 15) & ~(size_t)15);
+  const uint32_t* const* dcols = (const uint32_t* const*)dev;
   lcs.resize(max_log + 1);
-  for (uint32_t k = 0; k <= max_log; k++)
-    lcs[k] = {dcols.as<const uint32_t*>() + off[k], (uint32_t)by_layer[k].size(), 0};
-  dlcs = upload(lcs.data(), lcs.size() * sizeof(LayerCols));
+  for (uint32_t k = 0; k <= max_log; k++) lcs[k] = {dcols + off[k], (uint32_t)by_layer[k].size(), 0};
   outs.assign(layers, layers + max_log + 1);
-  douts = upload(outs.data(), outs.size() * sizeof(Hash*));
+  memcpy(host, flat.data(), flat.size() * sizeof(void*));
+  memcpy(host + o_lcs, lcs.data(), lcs.size() * sizeof(LayerCols));
+  memcpy(host + o_outs, outs.data(), outs.size() * sizeof(Hash*));
+  dlcs = (const LayerCols*)(dev + o_lcs);
+  douts = (Hash* const*)(dev + o_outs);
   // launch schedule, top-down
   steps.push_back({Step::LAYER, max_log, 1});
   int32_t top = (int32_t)max_log - 1;
@@ -228,8 +255,7 @@ void MerklePlan::enqueue() const {
       const uint32_t w_log = std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(8, s.top));
       const uint32_t blocks = 1u << (s.top - w_log);
       timed("merkle_subtree", [&] {
-        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts.as<Hash* const>(),
-                                               dlcs.as<const LayerCols>());
+        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs);
       });
     }
   }

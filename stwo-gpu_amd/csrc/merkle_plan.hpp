@@ -29,16 +29,30 @@ struct MerklePlan {
     uint32_t top, depth;
   };
   uint32_t max_log;
-  DevBuf dcols, dlcs, douts;
-  std::vector<LayerCols> lcs;
+  DevBuf own;                     // the device tables, unless they live in caller memory
+  const LayerCols* dlcs = nullptr;  // device: per-layer column tables
+  Hash* const* douts = nullptr;     // device: layer output pointers
+  std::vector<LayerCols> lcs;       // host copies (device column-table pointers)
   std::vector<Hash*> outs;
   std::vector<Step> steps;
 
+  // Bytes of device tables for a tree with n_cols columns and layers 0 .. max_log.
+  static size_t table_bytes(uint32_t n_cols, uint32_t max_log);
+
   // cols/logs: host arrays (device column pointers); layers: host array of max_log + 1
-  // device buffers, layers[k] holding 2^k hashes.
+  // device buffers, layers[k] holding 2^k hashes.  The tables are uploaded to a buffer
+  // of the plan's own.
   MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
              uint32_t max_log);
+  // Same, with the tables placed at device address `dev` (table_bytes long); their bytes
+  // are written to `host_image` and the caller uploads them to `dev` before enqueue().
+  MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
+             uint32_t max_log, void* dev, void* host_image);
   void enqueue() const;  // on ctx().cur; allocates nothing
+
+ private:
+  void build(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers, char* dev,
+             char* host);
 };
 
 }  // namespace merkle

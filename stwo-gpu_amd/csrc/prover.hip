@@ -183,6 +183,13 @@ namespace {
 
 uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
 
+// gather operations of one proof: fixed fields in chunks of 16 words, then per query
+// one operation per value word and per path hash
+uint64_t gather_ops_upper_bound(const Params& p, const Layout& L) {
+  const uint64_t fixed = L.per_query / 16 + 8 + 4 * (1ull << p.last_log) + 3 + L.n_line + (p.n_queries + 15) / 16;
+  return fixed + p.n_queries * L.query_words;
+}
+
 // arena layout of one job (byte offsets)
 struct Arena {
   uint64_t coeffs, lde, tree_t, dinv, powers, comp, comp_lde, tree_q, f1, f2, eap, quot, samples, jb, proof, end;
@@ -300,8 +307,26 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   for (uint32_t i = 0; i < C; i++) t.push_back(lde + i * 2 * n);
   for (uint32_t k = 0; k < 4; k++) t.push_back(comp + k * 2 * n);
   for (uint32_t k = 0; k < 4; k++) t.push_back(comp_lde + k * 4 * n);
-  tables_ = upload(t.data(), t.size() * sizeof(void*));
-  const uint32_t* const* tb = tables_.as<const uint32_t*>();
+  // one device region for every table of the job: pointer tables, the Merkle plans'
+  // tables and the gather table, filled as one host image and uploaded once
+  const uint32_t NL = L.n_line;
+  std::vector<uint64_t> o_tree(3 + NL);
+  uint64_t tb_bytes = align256(t.size() * sizeof(void*));
+  o_tree[0] = tb_bytes;
+  tb_bytes += align256(merkle::MerklePlan::table_bytes(C, ln + 1));
+  o_tree[1] = tb_bytes;
+  tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2));
+  for (uint32_t j = 1; j <= NL; j++) {
+    o_tree[1 + j] = tb_bytes;
+    tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2 - j));
+  }
+  const uint64_t o_ops = tb_bytes;
+  tb_bytes += align256(sizeof(GatherOp) * gather_ops_upper_bound(p, L));
+  tables_ = DevBuf(tb_bytes);
+  char* dtab = tables_.as<char>();
+  std::vector<char> image(tb_bytes);
+  memcpy(image.data(), t.data(), t.size() * sizeof(void*));
+  const uint32_t* const* tb = (const uint32_t* const*)dtab;
   t_coeffs = tb;
   t_lde = tb + C;
   t_comp = tb + 2 * C;
@@ -312,17 +337,20 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   layers_q = tree_layers(base + A.tree_q, ln + 2);
   {
     std::vector<uint32_t> logs(C, ln + 1);
-    tree_t = std::make_unique<merkle::MerklePlan>(C, t.data() + C, logs.data(), layers_t.data(), ln + 1);
+    tree_t = std::make_unique<merkle::MerklePlan>(C, t.data() + C, logs.data(), layers_t.data(), ln + 1,
+                                                  dtab + o_tree[0], image.data() + o_tree[0]);
     std::vector<uint32_t> l4(4, ln + 2);
-    tree_q = std::make_unique<merkle::MerklePlan>(4, t.data() + 2 * C + 4, l4.data(), layers_q.data(), ln + 2);
+    tree_q = std::make_unique<merkle::MerklePlan>(4, t.data() + 2 * C + 4, l4.data(), layers_q.data(), ln + 2,
+                                                  dtab + o_tree[1], image.data() + o_tree[1]);
   }
-  for (uint32_t j = 1; j <= L.n_line; j++) {
+  for (uint32_t j = 1; j <= NL; j++) {
     const uint32_t m = ln + 2 - j;
     layers_l.push_back(tree_layers(base + A.tree_l[j - 1], m));
     const uint32_t* cols[4];
     for (int k = 0; k < 4; k++) cols[k] = line[j - 1] + ((uint64_t)k << m);
     const uint32_t l4[4] = {m, m, m, m};
-    tree_l.push_back(std::make_unique<merkle::MerklePlan>(4, cols, l4, layers_l.back().data(), m));
+    tree_l.push_back(std::make_unique<merkle::MerklePlan>(4, cols, l4, layers_l.back().data(), m,
+                                                          dtab + o_tree[1 + j], image.data() + o_tree[1 + j]));
   }
 
   // proof assembly table
@@ -371,7 +399,10 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
     }
   }
   n_ops = (uint32_t)g.size();
-  ops = upload(g.data(), g.size() * sizeof(GatherOp));
+  require(n_ops <= gather_ops_upper_bound(p, L), "wide fibonacci prover: gather table overflow");
+  memcpy(image.data() + o_ops, g.data(), g.size() * sizeof(GatherOp));
+  ops_ = (const GatherOp*)(dtab + o_ops);
+  upload_to(tables_.p, image.data(), tb_bytes);
 
   host_proof = take_pinned(4 * L.total);
   hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
@@ -385,7 +416,6 @@ Job::~Job() {
   tree_t.reset();
   tree_q.reset();
   tree_l.clear();
-  ops = DevBuf();
   tables_ = DevBuf();
   arena = DevBuf();
 }
@@ -449,7 +479,7 @@ void Job::enqueue(uint64_t seed) {
   grind_async(jb->ch.digest, p.pow_bits, &jb->nonce);
   timed("channel", [&] { k_mix_nonce_queries<<<1, 1, 0, stream>>>(jb, p.n_queries, ln + 2); });
   timed("proof_gather", [&] {
-    k_gather<<<ceil_div(n_ops, 256), 256, 0, stream>>>(ops.as<GatherOp>(), n_ops, jb, proof);
+    k_gather<<<ceil_div(n_ops, 256), 256, 0, stream>>>(ops_, n_ops, jb, proof);
   });
   hip_check(hipMemcpyAsync(host_proof, proof, 4 * L.total, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
   hip_check(hipEventRecord(done, stream), "hipEventRecord");

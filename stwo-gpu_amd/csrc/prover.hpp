@@ -37,8 +37,8 @@ struct Job {
   DevBuf arena;  // every per-job device buffer
   std::unique_ptr<merkle::MerklePlan> tree_t, tree_q;
   std::vector<std::unique_ptr<merkle::MerklePlan>> tree_l;
-  DevBuf tables_;  // device pointer tables
-  DevBuf ops;      // gather table
+  DevBuf tables_;  // pointer tables, Merkle plan tables and the gather table
+  const struct GatherOp* ops_ = nullptr;
   uint32_t n_ops = 0;
   void* host_proof = nullptr;  // pinned
   hipEvent_t done = nullptr;

@@ -184,14 +184,18 @@ static Staging& take_staging(size_t bytes) {
   return c.staging.back();
 }
 
-DevBuf upload(const void* host, size_t bytes) {
-  DevBuf b(bytes);
-  if (!bytes) return b;
+void upload_to(void* dev, const void* host, size_t bytes) {
+  if (!bytes) return;
   Staging& st = take_staging(bytes);
   memcpy(st.host, host, bytes);
-  hip_check(hipMemcpyAsync(b.p, st.host, bytes, hipMemcpyHostToDevice, g_ctx.cur), "hipMemcpyAsync H2D");
+  hip_check(hipMemcpyAsync(dev, st.host, bytes, hipMemcpyHostToDevice, g_ctx.cur), "hipMemcpyAsync H2D");
   hip_check(hipEventRecord(st.done, g_ctx.cur), "hipEventRecord");
   st.busy = true;
+}
+
+DevBuf upload(const void* host, size_t bytes) {
+  DevBuf b(bytes);
+  upload_to(b.p, host, bytes);
   return b;
 }
 

@@ -128,8 +128,10 @@ struct DevBuf {
     return static_cast<T*>(p);
   }
 };
-// Copy a host array to a fresh pooled device buffer (stream-ordered).
+// Copy a host array to a fresh pooled device buffer (stream-ordered, through pinned staging).
 DevBuf upload(const void* host, size_t bytes);
+// Same into existing device memory.
+void upload_to(void* dev, const void* host, size_t bytes);
 
 // Launch bookkeeping: when kernel timing is on, bracket the launch with events.
 void launch_begin(const char* name);

@@ -137,3 +137,16 @@ extern "C" void evaluate_wide_fibonacci_constraint_quotients_on_domain(
     sync();
   });
 }
+
+extern "C" int stwo_hip_wide_fib_trace(uint32_t* const* columns, uint32_t n_cols, uint32_t log_n, uint64_t seed) {
+  return guarded([&] {
+    require(columns != nullptr, "wide fibonacci trace: null columns");
+    DevBuf table = upload(columns, (size_t)n_cols * sizeof(void*));
+    DevBuf s = upload(&seed, sizeof(seed));
+    wide_fib_trace_async(table.as<uint32_t*>(), n_cols, log_n, s.as<uint64_t>());
+  });
+}
+
+extern "C" int stwo_hip_vanishing_inverse(uint32_t* out, uint32_t log_ext, uint32_t log_n) {
+  return guarded([&] { vanishing_inverse_async(out, log_ext, log_n); });
+}

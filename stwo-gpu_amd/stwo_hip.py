@@ -124,6 +124,8 @@ def _declare(L):
         "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
         "stwo_hip_lde_schedule": (C.c_int, [u32, u32, u64, u32p, C.POINTER(C.c_int)]),
         "stwo_hip_grind": (C.c_int, [u32p, u32, C.POINTER(u64)]),
+        "stwo_hip_wide_fib_trace": (C.c_int, [pp, u32, u32, u64]),
+        "stwo_hip_vanishing_inverse": (C.c_int, [vp, u32, u32]),
         "stwo_hip_wide_fib_proof_size": (C.c_int, [C.POINTER(WideFibParams), C.POINTER(u64), C.POINTER(u64)]),
         "stwo_hip_wide_fib_prove": (C.c_int, [C.POINTER(WideFibParams), u64, u32p, u64]),
         "stwo_hip_server_create": (C.c_int, [C.POINTER(vp), u64, u32]),

@@ -88,3 +88,15 @@ def test_server_admits_by_memory_budget():
     spans.sort()
     for a, b in zip(spans, spans[1:]):
         assert b[0] >= a[1] - 1e-6, "two jobs ran together although only one fits the budget"
+
+
+@pytest.mark.parametrize("C,log_n,Q,pow_bits,last_log,seed", [(3, 4, 1, 0, 2, 1), (40, 6, 3, 6, 3, 2),
+                                                              (33, 11, 3, 8, 2, 21), (64, 13, 2, 10, 4, 22)])
+def test_plain_path_matches_device_prover(C, log_n, Q, pow_bits, last_log, seed):
+    """The reference-shaped per-operation path (synchronous part-1 ABI, host channel) and the
+    device prover produce the same proof."""
+    import wide_fib_plain
+    want = S.wide_fib_prove(S.wide_fib_params(C, log_n, Q, pow_bits, last_log), seed)
+    got = wide_fib_plain.prove(C, log_n, Q, pow_bits, last_log, seed)
+    diff = np.nonzero(got != want)[0]
+    assert diff.size == 0, f"first differing word {diff[:8]}"
