@@ -241,9 +241,14 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   }
 }
 
-void MerklePlan::enqueue() const {
+void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
   hipStream_t st = ctx().cur;
   for (const Step& s : steps) {
+    if (layer_ready) {  // the columns injected into the layers of this step must be written
+      const uint32_t lo = s.kind == Step::LAYER ? s.top : s.top + 1 - s.depth;
+      for (uint32_t k = lo; k <= s.top; k++)
+        if ((*layer_ready)[k]) hip_check(hipStreamWaitEvent(st, (*layer_ready)[k], 0), "hipStreamWaitEvent");
+    }
     if (s.kind == Step::LAYER) {
       const uint32_t k = s.top;
       const Hash* prev = k == max_log ? nullptr : outs[k + 1];

@@ -55,6 +55,7 @@ struct CommitPlan {
   std::unique_ptr<merkle::MerklePlan> merkle;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<hipEvent_t> class_done;  // one per class
 
   void enqueue_class(const Class& c) const {
     const void* const* base = c.tables.as<const void*>();
@@ -62,6 +63,11 @@ struct CommitPlan {
                        (uint32_t* const*)(base + 2 * c.n), c.n, c.log, log_blowup, tw, itw, tree_len, c.var);
   }
 
+  // Classes run on the auxiliary streams, largest first, round robin; each records an
+  // event when its LDE columns are written, and every Merkle launch waits only for the
+  // classes whose columns it hashes, so the top of the tree is built while smaller
+  // classes are still being transformed.  Every auxiliary stream's last event is waited
+  // for by some Merkle launch, which joins the streams back (as graph capture needs).
   void enqueue() const {
     Context& x = ctx();
     const int n_aux = (int)std::min<size_t>(classes.size(), N_AUX_STREAMS);
@@ -72,20 +78,19 @@ struct CommitPlan {
       } restore{x};
       hip_check(hipEventRecord(x.fork, x.stream), "hipEventRecord");
       for (int i = 0; i < n_aux; i++) hip_check(hipStreamWaitEvent(x.aux[i], x.fork, 0), "hipStreamWaitEvent");
-      // largest classes first, round robin over the auxiliary streams
+      std::vector<hipEvent_t> ready(merkle->max_log + 1, nullptr);
       for (size_t i = 0; i < classes.size(); i++) {
         x.cur = x.aux[i % n_aux];
         enqueue_class(classes[i]);
+        hip_check(hipEventRecord(class_done[i], x.cur), "hipEventRecord");
+        ready[classes[i].log + log_blowup] = class_done[i];
       }
       x.cur = x.stream;
-      for (int i = 0; i < n_aux; i++) {
-        hip_check(hipEventRecord(x.join[i], x.aux[i]), "hipEventRecord");
-        hip_check(hipStreamWaitEvent(x.stream, x.join[i], 0), "hipStreamWaitEvent");
-      }
+      merkle->enqueue(&ready);
     } else {
       for (const Class& c : classes) enqueue_class(c);
+      merkle->enqueue();
     }
-    merkle->enqueue();
   }
 
   void run(bool use_graph) {
@@ -151,6 +156,7 @@ struct CommitPlan {
   ~CommitPlan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : class_done) (void)hipEventDestroy(e);
   }
 };
 
@@ -196,6 +202,8 @@ extern "C" int stwo_hip_commit_plan_create(StwoCommitPlan** out, uint32_t n_cols
                           fft::lde_variant_for(kv.first, log_blowup, n)};
       p.classes.push_back(std::move(c));
     }
+    p.class_done.resize(p.classes.size());
+    for (auto& e : p.class_done) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     std::vector<uint32_t> lde_logs(n_cols);
     for (uint32_t i = 0; i < n_cols; i++) lde_logs[i] = log_sizes[i] + log_blowup;
     p.merkle = std::make_unique<merkle::MerklePlan>(n_cols, (const uint32_t* const*)ldes, lde_logs.data(),

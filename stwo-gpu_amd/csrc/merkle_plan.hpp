@@ -48,7 +48,11 @@ struct MerklePlan {
   // are written to `host_image` and the caller uploads them to `dev` before enqueue().
   MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
              uint32_t max_log, void* dev, void* host_image);
-  void enqueue() const;  // on ctx().cur; allocates nothing
+  // On ctx().cur; allocates nothing.  layer_ready (optional, indexed by layer): an event
+  // per layer (or null) that marks its injected columns written, e.g. on another stream;
+  // each step waits only for the layers it reads, so hashing starts while other
+  // columns are still being produced.
+  void enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr) const;
 
  private:
   void build(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers, char* dev,

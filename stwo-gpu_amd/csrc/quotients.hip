@@ -108,6 +108,37 @@ __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)
   return qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
 }
 
+constexpr int QG = 8;  // columns per load group
+
+template <int NB>
+__device__ __forceinline__ void absorb_group(uint64_t (&s)[NB][4], const uint32_t (&v)[QG], uint32_t h,
+                                             uint32_t n_heads, const ColHead* __restrict__ heads,
+                                             const ColEntry* __restrict__ ents) {
+#pragma unroll
+  for (int i = 0; i < QG; i++) {
+    if (h + i >= n_heads) break;
+    const ColHead H = heads[h + i];
+    for (uint32_t e = H.first; e < H.first + H.count; e++) {
+      const ColEntry E = ents[e];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (E.batch == (uint32_t)b) {
+          s[b][0] += (uint64_t)E.alpha.x.a * v[i];
+          s[b][1] += (uint64_t)E.alpha.x.b * v[i];
+          s[b][2] += (uint64_t)E.alpha.y.a * v[i];
+          s[b][3] += (uint64_t)E.alpha.y.b * v[i];
+          if (E.fold) {
+            s[b][0] = fold64(s[b][0]);
+            s[b][1] = fold64(s[b][1]);
+            s[b][2] = fold64(s[b][2]);
+            s[b][3] = fold64(s[b][3]);
+          }
+        }
+      }
+    }
+  }
+}
+
 template <int NB>
 __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
                                                               const uint32_t* const* __restrict__ cols,
@@ -121,33 +152,16 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
   uint64_t s[NB][4];
 #pragma unroll
   for (int b = 0; b < NB; b++) s[b][0] = s[b][1] = s[b][2] = s[b][3] = 0;
-  for (uint32_t h = 0; h < n_heads; h += 4) {
-    uint32_t v[4];
+  // software pipeline: the next group's column values are in flight while this group is absorbed
+  uint32_t v[QG], vn[QG];
 #pragma unroll
-    for (int i = 0; i < 4; i++) v[i] = h + i < n_heads ? cols[heads[h + i].col][row] : 0u;
+  for (int i = 0; i < QG; i++) v[i] = i < (int)n_heads ? cols[heads[i].col][row] : 0u;
+  for (uint32_t h = 0; h < n_heads; h += QG) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (h + i >= n_heads) break;
-      const ColHead H = heads[h + i];
-      for (uint32_t e = H.first; e < H.first + H.count; e++) {
-        const ColEntry E = ents[e];
+    for (int i = 0; i < QG; i++) vn[i] = h + QG + i < n_heads ? cols[heads[h + QG + i].col][row] : 0u;
+    absorb_group<NB>(s, v, h, n_heads, heads, ents);
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-          if (E.batch == (uint32_t)b) {
-            s[b][0] += (uint64_t)E.alpha.x.a * v[i];
-            s[b][1] += (uint64_t)E.alpha.x.b * v[i];
-            s[b][2] += (uint64_t)E.alpha.y.a * v[i];
-            s[b][3] += (uint64_t)E.alpha.y.b * v[i];
-            if (E.fold) {
-              s[b][0] = fold64(s[b][0]);
-              s[b][1] = fold64(s[b][1]);
-              s[b][2] = fold64(s[b][2]);
-              s[b][3] = fold64(s[b][3]);
-            }
-          }
-        }
-      }
-    }
+    for (int i = 0; i < QG; i++) v[i] = vn[i];
   }
   pt p = point_at_index(circle_domain_index(hi, hs, log_size - 1, brev(row, log_size)));
   qm31 acc = qzero();
