@@ -168,6 +168,8 @@ def bench_lde_commit(args):
                                "sample": f"{args.cpu_sample_cols} of the 256 columns (same seeds), full LDE + "
                                          f"Merkle tree over them; {dt:.2f} s",
                                "lde_column0_matches_gpu": parity}
+        if args.cpu_sample_cols == C:  # the whole workload: the roots must agree
+            out["cpu_baseline"]["root_matches_gpu"] = bool((layers[0].reshape(-1) == root_hash).all())
     return out
 
 
@@ -578,7 +580,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
-    ap.add_argument("--cpu-sample-cols", type=int, default=32)
+    ap.add_argument("--cpu-sample-cols", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=120)
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate (jobs/s)")

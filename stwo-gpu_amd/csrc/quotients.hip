@@ -88,17 +88,17 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
   o3[row] = acc.y.b;
 }
 
-// Column-major form for up to 4 batches: each column value of a row is loaded once and
-// added into the sums of every batch that samples the column (the batch-major kernel
-// above loads a column once per batch that samples it).  Columns are visited in groups
-// of four whose loads are issued together.  Host-planned folds keep each 64-bit sum
-// below 2^64 (at most three products between folds).
-struct ColHead {
-  uint32_t col, first, count, pad;  // entries [first, first + count) sample column col
-};
-struct ColEntry {
-  uint32_t batch, fold;  // fold: reduce this batch's sums after adding this product
-  qm31 alpha;
+// Column-major form for up to 4 batches: each sampled column value of a row is loaded
+// once and multiplied into the sums of all NB batches, with alpha = 0 for a batch that
+// does not sample the column.  Per column the host provides NB alphas (one uniform
+// 16 * NB-byte record), so the loop has no data-dependent branches; loads run one group
+// of QG columns ahead of the arithmetic.  Sums are folded after every third column
+// (three products < 2^64 on top of a folded sum).
+template <int NB>
+struct ColSlot {
+  const uint32_t* col;  // the column (device pointer)
+  uint32_t pad[2];
+  qm31 alpha[NB];
 };
 
 __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)[4], pt p, qm31 acc) {
@@ -108,43 +108,12 @@ __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)
   return qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
 }
 
-constexpr int QG = 8;  // columns per load group
-
-template <int NB>
-__device__ __forceinline__ void absorb_group(uint64_t (&s)[NB][4], const uint32_t (&v)[QG], uint32_t h,
-                                             uint32_t n_heads, const ColHead* __restrict__ heads,
-                                             const ColEntry* __restrict__ ents) {
-#pragma unroll
-  for (int i = 0; i < QG; i++) {
-    if (h + i >= n_heads) break;
-    const ColHead H = heads[h + i];
-    for (uint32_t e = H.first; e < H.first + H.count; e++) {
-      const ColEntry E = ents[e];
-#pragma unroll
-      for (int b = 0; b < NB; b++) {
-        if (E.batch == (uint32_t)b) {
-          s[b][0] += (uint64_t)E.alpha.x.a * v[i];
-          s[b][1] += (uint64_t)E.alpha.x.b * v[i];
-          s[b][2] += (uint64_t)E.alpha.y.a * v[i];
-          s[b][3] += (uint64_t)E.alpha.y.b * v[i];
-          if (E.fold) {
-            s[b][0] = fold64(s[b][0]);
-            s[b][1] = fold64(s[b][1]);
-            s[b][2] = fold64(s[b][2]);
-            s[b][3] = fold64(s[b][3]);
-          }
-        }
-      }
-    }
-  }
-}
+constexpr int QG = 6;  // columns per load group (a multiple of the fold period 3)
 
 template <int NB>
 __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
-                                                              const uint32_t* const* __restrict__ cols,
-                                                              const ColHead* __restrict__ heads, uint32_t n_heads,
-                                                              const ColEntry* __restrict__ ents,
-                                                              const Batch* __restrict__ batches,
+                                                              const ColSlot<NB>* __restrict__ slots,
+                                                              uint32_t n_slots, const Batch* __restrict__ batches,
                                                               uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
                                                               uint32_t* __restrict__ o2, uint32_t* __restrict__ o3) {
   uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,14 +121,35 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
   uint64_t s[NB][4];
 #pragma unroll
   for (int b = 0; b < NB; b++) s[b][0] = s[b][1] = s[b][2] = s[b][3] = 0;
-  // software pipeline: the next group's column values are in flight while this group is absorbed
+  // n_slots is padded to a multiple of QG with zero-alpha slots of column 0
   uint32_t v[QG], vn[QG];
 #pragma unroll
-  for (int i = 0; i < QG; i++) v[i] = i < (int)n_heads ? cols[heads[i].col][row] : 0u;
-  for (uint32_t h = 0; h < n_heads; h += QG) {
+  for (int i = 0; i < QG; i++) v[i] = slots[i].col[row];
+  for (uint32_t h = 0; h < n_slots; h += QG) {
+    if (h + QG < n_slots) {
 #pragma unroll
-    for (int i = 0; i < QG; i++) vn[i] = h + QG + i < n_heads ? cols[heads[h + QG + i].col][row] : 0u;
-    absorb_group<NB>(s, v, h, n_heads, heads, ents);
+      for (int i = 0; i < QG; i++) vn[i] = slots[h + QG + i].col[row];
+    }
+#pragma unroll
+    for (int i = 0; i < QG; i++) {
+      const ColSlot<NB>& S = slots[h + i];
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        s[b][0] += (uint64_t)S.alpha[b].x.a * v[i];
+        s[b][1] += (uint64_t)S.alpha[b].x.b * v[i];
+        s[b][2] += (uint64_t)S.alpha[b].y.a * v[i];
+        s[b][3] += (uint64_t)S.alpha[b].y.b * v[i];
+      }
+      if (i % 3 == 2) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          s[b][0] = fold64(s[b][0]);
+          s[b][1] = fold64(s[b][1]);
+          s[b][2] = fold64(s[b][2]);
+          s[b][3] = fold64(s[b][3]);
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < QG; i++) v[i] = vn[i];
   }
@@ -177,9 +167,11 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
 // point, sample values and random coefficient that live in device memory: lane j
 // takes samples j, j + 256, ...; the per-batch sums are reduced in LDS.
 constexpr uint32_t SB_NT = 256;
-__global__ __launch_bounds__(SB_NT) void single_batch_kernel(const qm31* __restrict__ point,
+__global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* const* __restrict__ cols,
+                                                             const qm31* __restrict__ point,
                                                              const qm31* __restrict__ values, uint32_t count,
-                                                             const qm31* __restrict__ rc_dev, Sample* ss, Batch* bs) {
+                                                             const qm31* __restrict__ rc_dev, ColSlot<1>* ss,
+                                                             uint32_t n_slots, Batch* bs) {
   __shared__ qm31 sa[SB_NT], sb[SB_NT];
   const qm31 px = point[0], py = point[1], rc = *rc_dev;
   const qm31 c = qsub(qm31{py.x, cneg(py.y)}, py);
@@ -191,8 +183,9 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const qm31* __restr
     const qm31 bb = qsub(qmul(v, c), qmul(a, py));
     la = qadd(la, qmul(alpha, a));
     lb = qadd(lb, qmul(alpha, bb));
-    ss[j] = Sample{j, 0, alpha};
+    ss[j] = ColSlot<1>{cols[j], {0, 0}, {alpha}};
   }
+  for (uint32_t j = count + threadIdx.x; j < n_slots; j += SB_NT) ss[j] = ColSlot<1>{cols[0], {0, 0}, {qzero()}};
   sa[threadIdx.x] = la;
   sb[threadIdx.x] = lb;
   __syncthreads();
@@ -208,8 +201,10 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const qm31* __restr
 
 }  // namespace quot
 
+static uint32_t padded_slots(uint32_t n_cols) { return (n_cols + quot::QG - 1) / quot::QG * quot::QG; }
 size_t quotient_single_point_scratch_bytes(uint32_t n_cols) {
-  return ((sizeof(quot::Sample) * std::max<uint32_t>(n_cols, 1) + 255) & ~(size_t)255) + sizeof(quot::Batch);
+  return ((sizeof(quot::ColSlot<1>) * padded_slots(std::max<uint32_t>(n_cols, 1)) + 255) & ~(size_t)255) +
+         sizeof(quot::Batch);
 }
 
 // accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
@@ -219,19 +214,69 @@ void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t 
                                              const qm31* d_values, const qm31* d_rc, void* d_scratch,
                                              uint32_t* const out[4]) {
   require(log_size >= 1 && log_size <= 31 && n_cols >= 1, "accumulate_quotients: bad sizes");
-  quot::Sample* ss = (quot::Sample*)d_scratch;
-  quot::Batch* bs = (quot::Batch*)((char*)d_scratch + ((sizeof(quot::Sample) * n_cols + 255) & ~(size_t)255));
+  const uint32_t n_slots = padded_slots(n_cols);
+  quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
+  quot::Batch* bs =
+      (quot::Batch*)((char*)d_scratch + ((sizeof(quot::ColSlot<1>) * n_slots + 255) & ~(size_t)255));
   timed("quotient_constants", [&] {
-    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_point, d_values, n_cols, d_rc, ss, bs);
+    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss, n_slots,
+                                                                bs);
   });
   timed("accumulate_quotients", [&] {
-    quot::accumulate_kernel<<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        hi, hs, log_size, d_cols, ss, bs, 1, out[0], out[1], out[2], out[3]);
+    quot::accumulate_cols_kernel<1><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
+        hi, hs, log_size, ss, n_slots, bs, out[0], out[1], out[2], out[3]);
   });
 }
 
+namespace {
+// column-major launch: per sampled column one slot with the alpha of every batch (0 when
+// the batch does not sample it; a column sampled twice in one batch gets the sum)
+template <int NB>
+void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,
+                    const std::vector<quot::Sample>& ss, const std::vector<quot::Batch>& bs, uint32_t* const out[4]) {
+  std::map<uint32_t, quot::ColSlot<NB>> by_col;
+  for (uint32_t b = 0; b < (uint32_t)NB; b++)
+    for (uint32_t j = 0; j < bs[b].count; j++) {
+      const quot::Sample& S = ss[bs[b].first + j];
+      auto it = by_col.find(S.col);
+      if (it == by_col.end()) {
+        quot::ColSlot<NB> z{};
+        z.col = h_cols[S.col];
+        for (int k = 0; k < NB; k++) z.alpha[k] = qzero();
+        it = by_col.emplace(S.col, z).first;
+      }
+      it->second.alpha[b] = qadd(it->second.alpha[b], S.alpha);
+    }
+  std::vector<quot::ColSlot<NB>> slots;
+  for (auto& kv : by_col) slots.push_back(kv.second);
+  quot::ColSlot<NB> pad{};
+  pad.col = slots.empty() ? h_cols[0] : slots[0].col;
+  for (int k = 0; k < NB; k++) pad.alpha[k] = qzero();
+  while (slots.empty() || slots.size() % quot::QG) slots.push_back(pad);
+  DevBuf dsl = upload(slots.data(), slots.size() * sizeof(quot::ColSlot<NB>));
+  DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
+  timed("accumulate_quotients", [&] {
+    quot::accumulate_cols_kernel<NB><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
+        hi, hs, log_size, dsl.as<quot::ColSlot<NB>>(), (uint32_t)slots.size(), dbs.as<quot::Batch>(), out[0], out[1],
+        out[2], out[3]);
+  });
+}
+void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,
+                 const std::vector<quot::Sample>& ss, const std::vector<quot::Batch>& bs, uint32_t n_batches,
+                 uint32_t* const out[4]) {
+  switch (n_batches) {
+    case 1: launch_cols_nb<1>(hi, hs, log_size, h_cols, ss, bs, out); break;
+    case 2: launch_cols_nb<2>(hi, hs, log_size, h_cols, ss, bs, out); break;
+    case 3: launch_cols_nb<3>(hi, hs, log_size, h_cols, ss, bs, out); break;
+    default: launch_cols_nb<4>(hi, hs, log_size, h_cols, ss, bs, out);
+  }
+}
+}  // namespace
+
 // points: n_batches x StwoCirclePointQM31 (host); indexes/values flattened in batch order (host).
+// d_cols: device table of the columns; h_cols: the same pointers in host memory.
 void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, const uint32_t* const* d_cols,
+                                const uint32_t* const* h_cols,
                                 qm31 rc, uint32_t n_batches, const StwoCirclePointQM31* points,
                                 const uint32_t* sizes, const uint32_t* col_idx, const StwoQM31* values,
                                 uint32_t* const out[4]) {
@@ -269,52 +314,7 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
     off += sizes[b];
   }
   if (n_batches >= 1 && n_batches <= 4) {
-    // column-major program: every sampled column once, with its (batch, alpha) entries
-    std::map<uint32_t, std::vector<std::pair<uint32_t, qm31>>> by_col;
-    for (uint32_t b = 0; b < n_batches; b++)
-      for (uint32_t j = 0; j < bs[b].count; j++) {
-        const quot::Sample& S = ss[bs[b].first + j];
-        by_col[S.col].push_back({b, S.alpha});
-      }
-    std::vector<quot::ColHead> heads;
-    std::vector<quot::ColEntry> ents;
-    uint32_t pending[4] = {0, 0, 0, 0};
-    for (auto& kv : by_col) {
-      heads.push_back(quot::ColHead{kv.first, (uint32_t)ents.size(), (uint32_t)kv.second.size(), 0});
-      for (auto& e : kv.second) {
-        const bool fold = ++pending[e.first] == 3;
-        if (fold) pending[e.first] = 0;
-        ents.push_back(quot::ColEntry{e.first, fold ? 1u : 0u, e.second});
-      }
-    }
-    DevBuf dh = upload(heads.data(), heads.size() * sizeof(quot::ColHead));
-    DevBuf de = upload(ents.data(), ents.size() * sizeof(quot::ColEntry));
-    DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
-    timed("accumulate_quotients", [&] {
-      const uint32_t blocks = ceil_div(domain_size, 256);
-      const uint32_t nh = (uint32_t)heads.size();
-      auto* h = dh.as<quot::ColHead>();
-      auto* e = de.as<quot::ColEntry>();
-      auto* b = dbs.as<quot::Batch>();
-      hipStream_t st = ctx().cur;
-      switch (n_batches) {
-        case 1:
-          quot::accumulate_cols_kernel<1><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
-                                                                    out[1], out[2], out[3]);
-          break;
-        case 2:
-          quot::accumulate_cols_kernel<2><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
-                                                                    out[1], out[2], out[3]);
-          break;
-        case 3:
-          quot::accumulate_cols_kernel<3><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
-                                                                    out[1], out[2], out[3]);
-          break;
-        default:
-          quot::accumulate_cols_kernel<4><<<blocks, 256, 0, st>>>(hi, hs, log_size, d_cols, h, nh, e, b, out[0],
-                                                                    out[1], out[2], out[3]);
-      }
-    });
+    launch_cols(hi, hs, log_size, h_cols, ss, bs, n_batches, out);
     return;
   }
   if (ss.empty()) ss.push_back(quot::Sample{0, 0, qzero()});
@@ -346,7 +346,7 @@ extern "C" int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_i
     DevBuf table = upload(columns, (size_t)n_columns * sizeof(void*));
     uint32_t* o[4] = {out[0], out[1], out[2], out[3]};
     accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
-                               table.as<const uint32_t*>(), qm31{{random_coeff.a, random_coeff.b},
+                               table.as<const uint32_t*>(), columns, qm31{{random_coeff.a, random_coeff.b},
                                                                  {random_coeff.c, random_coeff.d}},
                                n_batches, points, batch_sizes, column_indexes, values, o);
   });
@@ -368,8 +368,16 @@ extern "C" void accumulate_quotients(uint32_t half_coset_initial_index, uint32_t
     for (uint64_t i = 0; i < total; i++)
       require(sample_column_indexes[i] < number_of_columns, "accumulate_quotients: column index out of range");
     uint32_t* out[4] = {result_column_0, result_column_1, result_column_2, result_column_3};
+    // `columns` is a device table (copy_device_pointer_vec_from_host_to_device); keep a host copy
+    std::vector<const uint32_t*> host_cols(std::max<size_t>(number_of_columns, 1));
+    if (number_of_columns) {
+      hip_check(hipMemcpyAsync(host_cols.data(), columns, number_of_columns * sizeof(void*), hipMemcpyDeviceToHost,
+                               ctx().cur),
+                "hipMemcpyAsync D2H");
+      sync();
+    }
     accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
-                               (const uint32_t* const*)columns, qm31{{random_coeff.a, random_coeff.b},
+                               (const uint32_t* const*)columns, host_cols.data(), qm31{{random_coeff.a, random_coeff.b},
                                                                      {random_coeff.c, random_coeff.d}},
                                sample_size, (const StwoCirclePointQM31*)sample_points, sample_column_and_values_sizes,
                                sample_column_indexes, sample_column_values, out);
