@@ -243,11 +243,15 @@ def bench_cfg1(args):
     tw, itw = S.cached_twiddles(root)
     tree = S.MerkleTree(ldes, [L + 1] * C)
 
-    def step():
+    def step_calls():  # one asynchronous call per operation
         S.lde_columns_async(evals, coeffs, ldes, [L] * C, 1, tw, itw, 1 << root.log_size)
         tree.commit_async()
-    ms = _timed_steps(S, step, args.steps, args.warmup)
-    extra = {"root": "".join(f"{w:08x}" for w in tree.root())}
+    ms_calls = _timed_steps(S, step_calls, args.steps, args.warmup)
+    plan = S.CommitPlan(evals, coeffs, ldes, [L] * C, 1, tw, itw, 1 << root.log_size)
+    ms = _timed_steps(S, lambda: plan.run(True), args.steps, args.warmup)
+    assert (plan.root() == tree.root()).all()
+    extra = {"root": "".join(f"{w:08x}" for w in tree.root()), "launch": "hipGraph replay of the commit plan",
+             "ms_per_step_async_calls": round(ms_calls, 4)}
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -536,7 +540,9 @@ def bench_serving(args):
     for C, ln in shapes:
         warm.submit(S.wide_fib_params(C, ln, 3, pow_bits, last_log), 1)
     for _ in shapes:
-        assert warm.wait(60_000_000) is not None
+        r = warm.wait(60_000_000)
+        if r is None or r[1] != 0:
+            raise RuntimeError(f"warm-up job failed: {r and r[1]}: {S.last_error()}")
     warm.close()
 
     def summarize(done, n):

@@ -128,7 +128,7 @@ constexpr int SUB_MAX_DEPTH = 11;  // 1024 nodes in the first produced layer
 constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
 __global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
                                                       Hash* const* outs, const LayerCols* inj) {
-  __shared__ Hash s[1 << SUB_MAX_W_LOG];
+  extern __shared__ Hash s[];  // 2^w_log hashes (dynamic: shallow launches fit more blocks per CU)
   const uint32_t nodes0 = 1u << w_log;
   const uint32_t base0 = blockIdx.x << w_log;
   for (uint32_t d = 0; d < depth; d++) {
@@ -257,10 +257,11 @@ void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
       });
     } else {
       // at least 256 nodes of layer top per block (or the whole layer when it is smaller)
-      const uint32_t w_log = std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(8, s.top));
+      const uint32_t w_log = std::min(SUB_MAX_W_LOG, std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(8, s.top)));
       const uint32_t blocks = 1u << (s.top - w_log);
       timed("merkle_subtree", [&] {
-        subtree_kernel<<<blocks, 256, 0, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs);
+        subtree_kernel<<<blocks, 256, sizeof(Hash) << w_log, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts,
+                                                                  dlcs);
       });
     }
   }

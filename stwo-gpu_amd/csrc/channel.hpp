@@ -4,6 +4,7 @@
 // stwo_gpu_backend/src/backend.rs:14-20 binds to the Blake2s Merkle channel); the
 // transcript byte layout below is this library's own:
 //   mix(words):   digest <- B2S(digest || words), n_draws <- 0
+//   mix_felts(v): mix(tree_root(words of v)) (tree hash of blake2s.hpp, QM31 as 4 words)
 //   draw_words(): B2S(digest || n_draws || 0), then n_draws += 1   (8 words)
 //   draw_qm31():  first 4 words of a draw, redrawn unless all are < 2P, each reduced mod P
 //   draw_point(): t = draw_qm31(), (x, y) = ((1 - t^2) / (1 + t^2), 2t / (1 + t^2))
@@ -32,16 +33,8 @@ struct Channel {
     hs.finish(digest);
     n_draws = 0;
   }
-  __host__ __device__ void mix_qm31(const qm31* v, uint32_t n) {
-    b2s::Hasher hs;
-    hs.words(digest, 8);
-    for (uint32_t i = 0; i < n; i++) {
-      const uint32_t w[4] = {v[i].x.a, v[i].x.b, v[i].y.a, v[i].y.b};
-      hs.words(w, 4);
-    }
-    hs.finish(digest);
-    n_draws = 0;
-  }
+  // Field-element vectors are mixed through their tree hash (blake2s.hpp): the prover
+  // computes the root in parallel (prover.hip) and mixes it with mix(root, 8).
   __host__ __device__ void draw_words(uint32_t* out) {
     uint32_t m[10];
     for (int i = 0; i < 8; i++) m[i] = digest[i];

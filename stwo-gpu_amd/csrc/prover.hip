@@ -103,23 +103,90 @@ __global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root) {
   jb->ch.mix(root->s, 8);
   jb->ch.draw_point(jb->z[0], jb->z[1]);
 }
-__global__ void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n) {
-  jb->ch.mix_qm31(samples, n);
-  jb->beta = jb->ch.draw_qm31();
-  jb->fri_alpha[0] = jb->ch.draw_qm31();
-}
+// Tree root (blake2s.hpp) of a QM31 vector held as AoS (soa == false: v[i] = 4 words) or as
+// four coordinate columns, by one block: leaves from global memory, levels in LDS.
+constexpr uint32_t TREE_NT = 256;
+constexpr uint32_t TREE_MAX_LEAVES = 1040;  // 4 words per element: up to 4160 elements
 struct Soa4c {
   const uint32_t* c[4];
 };
-__global__ void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
-  b2s::Hasher hs;
-  hs.words(jb->ch.digest, 8);
-  for (uint32_t i = 0; i < n; i++) {
-    const uint32_t w[4] = {last.c[0][i], last.c[1][i], last.c[2][i], last.c[3][i]};
-    hs.words(w, 4);
+__device__ void block_tree_root(const qm31* aos, Soa4c soa, bool use_soa, uint32_t n, uint32_t (&root)[8]) {
+  __shared__ merkle::Hash buf[TREE_MAX_LEAVES];
+  const uint32_t n_words = 4 * n, nb = (n_words + 15) / 16;
+  for (uint32_t i = threadIdx.x; i < nb; i += TREE_NT) {
+    uint32_t m[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t w = 16 * i + j, e = w >> 2, k = w & 3;
+      uint32_t x = 0;
+      if (w < n_words) {
+        if (use_soa) {
+          x = soa.c[k][e];
+        } else {
+          const qm31 v = aos[e];
+          x = k == 0 ? v.x.a : k == 1 ? v.x.b : k == 2 ? v.y.a : v.y.b;
+        }
+      }
+      m[j] = x;
+    }
+    uint32_t h[8];
+    b2s::f0(m, h);
+#pragma unroll
+    for (int j = 0; j < 8; j++) buf[i].s[j] = h[j];
   }
-  hs.finish(jb->ch.digest);
-  jb->ch.n_draws = 0;
+  __syncthreads();
+  // each level: nodes are computed into registers (at most PAIRS per lane), then written
+  // back after a barrier, so no lane overwrites a node another lane still reads
+  constexpr uint32_t PAIRS = (TREE_MAX_LEAVES / 2 + TREE_NT - 1) / TREE_NT;
+  uint32_t cnt = nb;
+  while (cnt > 1) {
+    const uint32_t half = (cnt + 1) / 2;
+    uint32_t nodes[PAIRS][8];
+#pragma unroll
+    for (uint32_t k = 0; k < PAIRS; k++) {
+      const uint32_t i = threadIdx.x + k * TREE_NT;
+      if (i >= half) break;
+      if (2 * i + 1 < cnt) {
+        uint32_t m[16];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          m[j] = buf[2 * i].s[j];
+          m[8 + j] = buf[2 * i + 1].s[j];
+        }
+        b2s::f0(m, nodes[k]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) nodes[k][j] = buf[2 * i].s[j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < PAIRS; k++) {
+      const uint32_t i = threadIdx.x + k * TREE_NT;
+      if (i >= half) break;
+#pragma unroll
+      for (int j = 0; j < 8; j++) buf[i].s[j] = nodes[k][j];
+    }
+    __syncthreads();
+    cnt = half;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) root[j] = buf[0].s[j];
+}
+
+__global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n) {
+  uint32_t root[8];
+  block_tree_root(samples, Soa4c{}, false, n, root);
+  if (threadIdx.x == 0) {
+    jb->ch.mix(root, 8);
+    jb->beta = jb->ch.draw_qm31();
+    jb->fri_alpha[0] = jb->ch.draw_qm31();
+  }
+}
+__global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
+  uint32_t root[8];
+  block_tree_root(nullptr, last, true, n, root);
+  if (threadIdx.x == 0) jb->ch.mix(root, 8);
 }
 __global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size) {
   const uint32_t w[2] = {(uint32_t)jb->nonce, (uint32_t)(jb->nonce >> 32)};
@@ -425,12 +492,12 @@ void Job::enqueue(uint64_t seed) {
   const uint32_t C = p.n_cols, ln = p.log_n;
   const uint64_t n = 1ull << ln;
 
-  timed("channel", [&] { k_begin<<<1, 1, 0, stream>>>(jb, seed, p, L.n_line); });
+  timed("channel_begin", [&] { k_begin<<<1, 1, 0, stream>>>(jb, seed, p, L.n_line); });
   wide_fib_trace_async((uint32_t* const*)t_coeffs, C, ln, &jb->seed);
   fft::lde_same_size(t_coeffs, (uint32_t* const*)t_coeffs, (uint32_t* const*)t_lde, C, ln, 1, tw, itw, tree_len,
                      fft::lde_variant_for(ln, 1, C));
   tree_t->enqueue();
-  timed("channel", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
+  timed("channel_trace_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
 
   qm31_powers_async(&jb->alpha, C - 2, powers);
   vanishing_inverse_async(dinv, ln + 1, ln);
@@ -439,11 +506,11 @@ void Job::enqueue(uint64_t seed) {
   fft::lde_same_size(t_comp, (uint32_t* const*)t_comp, (uint32_t* const*)t_comp_lde, 4, ln + 1, 1, tw, itw, tree_len,
                      fft::lde_variant_for(ln + 1, 1, 4));
   tree_q->enqueue();
-  timed("channel", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
+  timed("channel_oods_point", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
 
   eval_at_point_columns_async(t_coeffs, C, ln, jb->z, samples, eap_scratch);
   eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch);
-  timed("channel", [&] { k_mix_samples_draw<<<1, 1, 0, stream>>>(jb, samples, C + 4); });
+  timed("channel_samples", [&] { k_mix_samples_draw<<<1, TREE_NT, 0, stream>>>(jb, samples, C + 4); });
 
   uint32_t* f1c[4] = {f1, f1 + 2 * n, f1 + 4 * n, f1 + 6 * n};
   uint32_t* f2c[4] = {f2, f2 + 4 * n, f2 + 8 * n, f2 + 12 * n};
@@ -463,7 +530,7 @@ void Job::enqueue(uint64_t seed) {
   for (uint32_t j = 1; j <= L.n_line; j++) {
     const uint32_t m = ln + 2 - j;
     tree_l[j - 1]->enqueue();
-    timed("channel", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
+    timed("channel_fri_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
     auto src = coords(line[j - 1], m), dst = coords(line[j], m - 1);
     fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j]);
     if (j == 1)
@@ -474,10 +541,10 @@ void Job::enqueue(uint64_t seed) {
     const uint32_t nl = 1u << p.last_log;
     const uint32_t* last = line[L.n_line];
     Soa4c s{{last, last + nl, last + 2 * nl, last + 3 * nl}};
-    timed("channel", [&] { k_mix_last<<<1, 1, 0, stream>>>(jb, s, nl); });
+    timed("channel_last_layer", [&] { k_mix_last<<<1, TREE_NT, 0, stream>>>(jb, s, nl); });
   }
   grind_async(jb->ch.digest, p.pow_bits, &jb->nonce);
-  timed("channel", [&] { k_mix_nonce_queries<<<1, 1, 0, stream>>>(jb, p.n_queries, ln + 2); });
+  timed("channel_queries", [&] { k_mix_nonce_queries<<<1, 1, 0, stream>>>(jb, p.n_queries, ln + 2); });
   timed("proof_gather", [&] {
     k_gather<<<ceil_div(n_ops, 256), 256, 0, stream>>>(ops_, n_ops, jb, proof);
   });

@@ -138,6 +138,9 @@ void launch_begin(const char* name);
 void launch_end(const char* name);
 template <class F>
 inline void timed(const char* name, F&& f) {
+  // an error left by an earlier call on this thread is reported as such, not as this launch's
+  const hipError_t before = hipGetLastError();
+  if (before != hipSuccess) fail(STWO_HIP_ERR_HIP, "%s: earlier error pending: %s", name, hipGetErrorString(before));
   launch_begin(name);
   f();
   hip_check(hipGetLastError(), name);

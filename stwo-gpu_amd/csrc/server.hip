@@ -17,6 +17,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -45,6 +46,7 @@ struct Result {
   int status;
   std::vector<uint32_t> proof;
   double t_submit, t_start, t_end, host_s;
+  std::string error;  // the worker's error message when status != OK
 };
 
 struct Server {
@@ -68,12 +70,16 @@ struct Server {
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
   void finish(Running& r, int status) {
-    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now(), r.host_s};
+    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now(), r.host_s,
+               status == STWO_HIP_OK ? std::string() : std::string("job failed on the device")};
     if (status == STWO_HIP_OK) {
       const uint32_t* pr = r.job->result();
       res.proof.assign(pr, pr + r.job->L.total);
       // no proof-of-work nonce found within the search bound
-      if (pr[r.job->L.nonce] == 0xffffffffu && pr[r.job->L.nonce + 1] == 0xffffffffu) res.status = STWO_HIP_ERR_ARG;
+      if (pr[r.job->L.nonce] == 0xffffffffu && pr[r.job->L.nonce + 1] == 0xffffffffu) {
+        res.status = STWO_HIP_ERR_ARG;
+        res.error = "no proof-of-work nonce within the search bound";
+      }
     }
     {
       auto g = lock();
@@ -128,7 +134,7 @@ struct Server {
         }
         if (st != STWO_HIP_OK) {
           std::lock_guard<std::mutex> l(m);
-          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now(), 0.0});
+          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now(), 0.0, stwo_hip_last_error()});
           cv_done.notify_all();
           continue;
         }
@@ -241,6 +247,7 @@ extern "C" int stwo_hip_server_wait(StwoProofServer* h, uint32_t timeout_us, uin
     l.unlock();
     if (job_id) *job_id = r.id;
     if (job_status) *job_status = r.status;
+    if (r.status != STWO_HIP_OK) set_error(r.status, "server job %llu: %s", (unsigned long long)r.id, r.error.c_str());
     if (proof_words) *proof_words = r.proof.size();
     if (proof && proof_capacity_words >= r.proof.size()) std::copy(r.proof.begin(), r.proof.end(), proof);
     if (times) {

@@ -165,6 +165,11 @@ def check(status=None):
         raise StwoHipError(f"stwo_hip error {st}: {msg}")
 
 
+def last_error():
+    """Message of the last failure on the calling thread ("" when none)."""
+    return lib().stwo_hip_last_error().decode()
+
+
 def _c(status):
     check(status)
 

@@ -16,7 +16,6 @@ It is the baseline the proof server is measured against, and a second, independe
 assembly of the protocol of csrc/prover.hip: its proofs must equal the device prover's
 word for word.
 """
-import ctypes as C
 import hashlib
 import struct
 
@@ -29,10 +28,6 @@ MAGIC, VERSION, TAG = 0x50575453, 1, 0x57464942
 
 
 # ------------------------------------------------------------------ QM31 on the host
-def _cm_mul(a, b):
-    return ((a[0] * b[0] - a[1] * b[1]) % P, (a[0] * b[1] + a[1] * b[0]) % P)
-
-
 def qm_mul(a, b):
     return S.qm_mul(a, b)
 
@@ -87,6 +82,56 @@ class Channel:
                 if len(out) < n:
                     out.append(w & ((1 << log_size) - 1))
         return out
+
+
+_SIGMA = [
+    (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15), (14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3),
+    (11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4), (7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8),
+    (9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13), (2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9),
+    (12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11), (13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10),
+    (6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5), (10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0)]
+_IV = (0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19)
+_M32 = 0xFFFFFFFF
+
+
+def _f0(m):
+    """BLAKE2s compression from the zero state, zero counter and flags (RFC 7693)."""
+    v = [0] * 8 + list(_IV)
+
+    def rot(x, c):
+        return ((x >> c) | (x << (32 - c))) & _M32
+
+    def g(a, b, c, d, x, y):
+        v[a] = (v[a] + v[b] + x) & _M32
+        v[d] = rot(v[d] ^ v[a], 16)
+        v[c] = (v[c] + v[d]) & _M32
+        v[b] = rot(v[b] ^ v[c], 12)
+        v[a] = (v[a] + v[b] + y) & _M32
+        v[d] = rot(v[d] ^ v[a], 8)
+        v[c] = (v[c] + v[d]) & _M32
+        v[b] = rot(v[b] ^ v[c], 7)
+    for s in _SIGMA:
+        g(0, 4, 8, 12, m[s[0]], m[s[1]])
+        g(1, 5, 9, 13, m[s[2]], m[s[3]])
+        g(2, 6, 10, 14, m[s[4]], m[s[5]])
+        g(3, 7, 11, 15, m[s[6]], m[s[7]])
+        g(0, 5, 10, 15, m[s[8]], m[s[9]])
+        g(1, 6, 11, 12, m[s[10]], m[s[11]])
+        g(2, 7, 8, 13, m[s[12]], m[s[13]])
+        g(3, 4, 9, 14, m[s[14]], m[s[15]])
+    return [v[i] ^ v[i + 8] for i in range(8)]
+
+
+def tree_root(words):
+    """Tree hash of a word sequence (csrc/blake2s.hpp): zero-padded 16-word leaves, pairwise
+    levels, an unpaired last node carried up."""
+    nodes = []
+    for i in range(0, len(words), 16):
+        blk = [int(w) for w in words[i:i + 16]]
+        nodes.append(_f0(blk + [0] * (16 - len(blk))))
+    while len(nodes) > 1:
+        nodes = [_f0(nodes[i] + nodes[i + 1]) if i + 1 < len(nodes) else nodes[i] for i in range(0, len(nodes), 2)]
+    return nodes[0]
 
 
 def pow_ok(digest, nonce, bits):
@@ -199,7 +244,7 @@ def prove(n_cols, log_n, n_queries, pow_bits, last_log, seed):
     ch.mix(_root(layers_q))
     zx, zy = ch.draw_point()
     samples = [S.eval_at_point(c, zx, zy) for c in trace] + [S.eval_at_point(c, zx, zy) for c in comp.columns]
-    ch.mix([w for v in samples for w in v])
+    ch.mix(tree_root([w for v in samples for w in v]))
     beta = ch.draw_qm31()
     alphas = [ch.draw_qm31()]
 
@@ -221,7 +266,7 @@ def prove(n_cols, log_n, n_queries, pow_bits, last_log, seed):
             S.fold_circle_into_line(nxt, f1, alphas[1], tree)
         lines.append(nxt)
     last = lines[-1].to_numpy()
-    ch.mix([int(last[k][i]) for i in range(1 << last_log) for k in range(4)])
+    ch.mix(tree_root([int(last[k][i]) for i in range(1 << last_log) for k in range(4)]))
     nonce = 0
     while not pow_ok(ch.digest, nonce, pow_bits):
         nonce += 1

@@ -46,7 +46,8 @@ class Channel:
         self.n_draws = 0
 
     def mix_qm31(self, vals):
-        self.mix([c for v in vals for c in v])
+        """Field-element vectors are mixed through their tree hash (tree_root)."""
+        self.mix(tree_root([int(c) for v in vals for c in v]))
 
     def draw_words(self):
         out = b2s(self.digest + [self.n_draws, 0])
@@ -76,6 +77,19 @@ class Channel:
                 if len(out) < n:
                     out.append(w & ((1 << log_size) - 1))
         return out
+
+
+def tree_root(words):
+    """16-word leaves (last zero-padded) hashed with the zero-state compression, then
+    pairwise levels, an unpaired last node carried up unchanged."""
+    nodes = []
+    for i in range(0, len(words), 16):
+        blk = list(words[i:i + 16])
+        nodes.append(_compress([0] * 8, blk + [0] * (16 - len(blk))))
+    while len(nodes) > 1:
+        nodes = [_compress([0] * 8, nodes[i] + nodes[i + 1]) if i + 1 < len(nodes) else nodes[i]
+                 for i in range(0, len(nodes), 2)]
+    return nodes[0]
 
 
 def pow_zeros(digest, nonce):
