@@ -172,6 +172,7 @@ static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 // compressions); it gets its own launch with one lane per node, so all of its nodes
 // run in parallel.  Runs of layers with few columns are built by subtree launches.
 constexpr uint32_t HEAVY_CHUNKS = 2;
+constexpr int32_t WIDE_LAYER = 16;
 
 size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log) {
   const size_t cols = ((size_t)std::max<uint32_t>(n_cols, 1) * sizeof(void*) + 15) & ~(size_t)15;
@@ -223,11 +224,13 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   memcpy(host + o_outs, outs.data(), outs.size() * sizeof(Hash*));
   dlcs = (const LayerCols*)(dev + o_lcs);
   douts = (Hash* const*)(dev + o_outs);
-  // launch schedule, top-down
+  // launch schedule, top-down: wide layers (more than 2^16 nodes, enough lanes to fill
+  // the chip) and injection-heavy layers get one launch each, runs of the remaining
+  // layers are built by subtree launches
   steps.push_back({Step::LAYER, max_log, 1});
   int32_t top = (int32_t)max_log - 1;
   while (top >= 0) {
-    if ((lcs[top].n_cols + 15) / 16 >= HEAVY_CHUNKS && top >= 8) {
+    if (top > WIDE_LAYER || ((lcs[top].n_cols + 15) / 16 >= HEAVY_CHUNKS && top >= 8)) {
       steps.push_back({Step::LAYER, (uint32_t)top, 1});
       top--;
       continue;
