@@ -199,6 +199,16 @@ def _kernel_avg(S, step, names, reps=3):
     return res
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def _cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
 def _line(metric, value, unit, ms, args, config, roofline=None, extra=None, higher=True):
     out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
@@ -281,9 +291,19 @@ def bench_batch_inverse(args):
     ms = _timed_steps(S, step, args.steps, args.warmup)
     ks = _kernel_avg(S, step, ["batch_inverse_qm31"])
     avg = ks["batch_inverse_qm31"]["avg_ms"]
+    extra = {"kernels": ks, "note": "ms_per_step is the synchronous ABI call (launch + sync)"}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        host = O.fill_uniform_m31(4 * n, 0x5EED0002, 1 << 40)
+        t0 = time.perf_counter()
+        want = O.batch_inverse_qm31(host)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": n / dt, "unit": "elems/s", "kind": "port", "sample": f"whole input; {dt:.2f} s",
+                                 "cores": _cores()}
+        extra["parity_matches_oracle"] = bool((dst.to_numpy().reshape(-1) == want.reshape(-1)).all())
     return _line("batch_inverse QM31 elems/s", n / (ms * 1e-3), "elems/s", ms, args,
-                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"}, _roof(32 * n, avg, "batch_inverse_qm31", "batch_inverse_qm31_2e24"),
-                 {"kernels": ks, "note": "ms_per_step is the synchronous ABI call (launch + sync)"})
+                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"},
+                 _roof(32 * n, avg, "batch_inverse_qm31", "batch_inverse_qm31_2e24"), extra)
 
 
 def bench_fri(args):
@@ -311,10 +331,31 @@ def bench_fri(args):
     circle_bytes = 16 * (1 << L) + 2 * 16 * (1 << (L - 1)) + 4 * (1 << (L - 2))
     line_bytes = sum(16 * (1 << l) + 16 * (1 << (l - 1)) + 4 * (1 << (l - 1)) for l in range(L - 1, 4, -1))
     total = circle_bytes + line_bytes
+    extra = {"kernels": ks, "algorithmic_bytes_per_step": total}
+    if not args.no_cpu_baseline:
+        # one fresh pass on the device (the timed steps accumulate into layer 0) against the oracle
+        layers[0] = S.SecureColumn.zeros(1 << (L - 1))
+        step()
+        S.synchronize()
+        O = _oracle()
+        host = np.stack([O.fill_uniform_m31(1 << L, 0x5EED0002, (2 << 40) + k * (1 << L)) for k in range(4)])
+        t0 = time.perf_counter()
+        cur = O.fold_circle_into_line(np.zeros((4, 1 << (L - 1)), np.uint32), host, root.initial_index,
+                                      root.step_size, alphas[0])
+        d = S.Coset(root.initial_index, root.step_size, root.log_size)
+        outs = [cur]
+        for i in range(1, len(layers)):
+            cur = O.fold_line(cur, d.initial_index, d.step_size, alphas[i])
+            d = d.double()
+            outs.append(cur)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": total / dt / 1e9, "unit": "GB/s", "kind": "port",
+                                 "sample": f"whole fold chain; {dt:.3f} s", "cores": _cores()}
+        extra["parity_matches_oracle"] = bool(all((layers[i].to_numpy() == outs[i]).all() for i in range(len(layers))))
     return _line("FRI fold chain GB/s (log 22 circle -> line -> log 4)", total / (ms * 1e-3) / 1e9, "GB/s", ms, args,
                  {"workload": "cfg2 FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
-                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line", "fri_fold_2e22_to_2e4"),
-                 {"kernels": ks, "algorithmic_bytes_per_step": total})
+                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line",
+                       "fri_fold_2e22_to_2e4"), extra)
 
 
 def bench_quotients(args):
@@ -346,9 +387,20 @@ def bench_quotients(args):
     ks = _kernel_avg(S, step, ["accumulate_quotients"])
     alg = 4 * C * n + 16 * n
     n_samples = sum(len(cv) for _, cv in batches)
+    extra = {"kernels": ks}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        host = [O.fill_uniform_m31(n, 0x5EED0003, c * n) for c in range(C)]
+        t0 = time.perf_counter()
+        want = O.accumulate_quotients(dom.initial_index, dom.step_size, L, host, rc, batches)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": n / dt, "unit": "rows/s", "kind": "port", "sample": f"whole workload; {dt:.2f} s",
+                                 "cores": _cores()}
+        extra["parity_matches_oracle"] = bool((out.to_numpy() == want).all())
     return _line("accumulate_quotients rows/s", n / (ms * 1e-3), "rows/s", ms, args,
                  {"workload": f"cfg3 quotients: {C} M31 columns log {L}, 3 QM31 points, {n_samples} samples"},
-                 _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients", "quotients_200x2e21_3pts"), {"kernels": ks})
+                 _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients", "quotients_200x2e21_3pts"),
+                 extra)
 
 
 def cfg4_log_sizes(n_cols=1200, seed=0x5EED0004):

@@ -37,6 +37,11 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* every declaration below is the library's exported interface (it is built with hidden
+ * visibility for everything else) */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
 
 /* bindings.rs:8-14 (CudaSecureField): (a + b i) + (c + d i) u */
 typedef struct { uint32_t a, b, c, d; } StwoQM31;
@@ -319,6 +324,9 @@ int stwo_hip_kernel_timing(int enable);
 int stwo_hip_kernel_stats(const char *kernel_name, uint64_t *launches, double *total_ms);
 int stwo_hip_kernel_stats_reset(void);
 
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
 #ifdef __cplusplus
 }
 #endif

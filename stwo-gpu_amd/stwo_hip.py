@@ -150,6 +150,7 @@ def exported_symbols():
     import re
     hdr = open(os.path.join(os.path.dirname(HERE), "include", "stwo_hip.h")).read()
     hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    hdr = "\n".join(l for l in hdr.splitlines() if not l.lstrip().startswith("#"))  # preprocessor lines
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", hdr)
     skip = {"if", "for", "while", "sizeof", "return"}
     return sorted({n for n in names if n not in skip and not n.isupper()})
