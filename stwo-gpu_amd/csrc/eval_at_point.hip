@@ -99,6 +99,33 @@ __global__ __launch_bounds__(NT) void first_pass_cols(const uint32_t* const* __r
   if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
 }
 
+// Columns of log >= 12: block (b, col) folds coefficients [b 2^12, (b+1) 2^12) of column
+// col; each lane folds 16 consecutive coefficients in registers (bits 0..3: factors
+// f[L-1] .. f[L-4]) from four 16-byte loads, then the 256 lane values are folded in LDS.
+__global__ __launch_bounds__(NT) void first_pass_cols16(const uint32_t* const* __restrict__ cols, uint32_t L,
+                                                        const Factors* __restrict__ F, qm31* partial) {
+  __shared__ qm31 s[NT];
+  const uint4* c = reinterpret_cast<const uint4*>(cols[blockIdx.y] + ((size_t)blockIdx.x << 12) + 16 * threadIdx.x);
+  const uint4 q0 = c[0], q1 = c[1], q2 = c[2], q3 = c[3];
+  const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  const qm31 f0 = F->f[L - 1], f1 = F->f[L - 2], f2 = F->f[L - 3], f3 = F->f[L - 4];
+  qm31 a[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    a[j] = qscale(f0, v[2 * j + 1]);
+    a[j].x.a = add(a[j].x.a, v[2 * j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) a[j] = qadd(a[2 * j], qmul(a[2 * j + 1], f1));
+#pragma unroll
+  for (int j = 0; j < 2; j++) a[j] = qadd(a[2 * j], qmul(a[2 * j + 1], f2));
+  s[threadIdx.x] = qadd(a[0], qmul(a[1], f3));
+  __syncthreads();
+  tree_reduce(s, NT, *F, (int)L - 5);
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
 __global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g, const Factors* __restrict__ F,
                                                       int fi, qm31* out) {
   __shared__ qm31 s[1 << (CB - 1)];
@@ -112,7 +139,7 @@ __global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g
 }  // namespace eap
 
 size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L) {
-  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
+  const uint32_t cb = L >= 12 ? 12 : std::min<uint32_t>(L, eap::CB);
   return 512 + 2 * sizeof(qm31) * (size_t)n_cols * (1ull << (L - cb));
 }
 
@@ -121,14 +148,17 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
                                  qm31* d_out, void* d_scratch) {
   require(L >= 1 && L <= 31 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
   eap::Factors* F = (eap::Factors*)d_scratch;
-  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
+  const uint32_t cb = L >= 12 ? 12 : std::min<uint32_t>(L, eap::CB);
   uint32_t blocks = 1u << (L - cb);
   qm31* pa = (qm31*)((char*)d_scratch + 512);
   qm31* pb = pa + (size_t)n_cols * blocks;
   timed("eval_at_point", [&] {
     eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
-    eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
-                                                                           blocks == 1 ? d_out : pa);
+    if (cb == 12)
+      eap::first_pass_cols16<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, F, blocks == 1 ? d_out : pa);
+    else
+      eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
+                                                                             blocks == 1 ? d_out : pa);
   });
   int fi = (int)(L - cb) - 1;
   while (blocks > 1) {

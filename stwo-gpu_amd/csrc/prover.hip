@@ -250,6 +250,13 @@ namespace {
 
 uint64_t align256(uint64_t b) { return (b + 255) & ~255ull; }
 
+// Column stride (words) inside the arena.  Columns of one table are read together at the
+// same row (Merkle leaves, composition, quotients); an exact power-of-two stride would
+// send all of those reads to the same HBM channels, so each column is padded by
+// 4 KiB + 256 B (keeping 16-byte alignment).
+constexpr uint64_t COL_PAD = 1088;
+uint64_t cs(uint64_t len) { return len + COL_PAD; }
+
 // gather operations of one proof: fixed fields in chunks of 16 words, then per query
 // one operation per value word and per path hash
 uint64_t gather_ops_upper_bound(const Params& p, const Layout& L) {
@@ -269,17 +276,17 @@ struct Arena {
       o += align256(bytes);
       return at;
     };
-    coeffs = take(4 * C * n);
-    lde = take(8 * C * n);
+    coeffs = take(4 * C * cs(n));
+    lde = take(4 * C * cs(2 * n));
     tree_t = take(32 * ((4 * n) - 1));  // layers 0 .. log_n + 1
     dinv = take(8 * n);
     powers = take(16 * (C - 2));
-    comp = take(4 * 4 * 2 * n);
-    comp_lde = take(4 * 4 * 4 * n);
+    comp = take(4 * 4 * cs(2 * n));
+    comp_lde = take(4 * 4 * cs(4 * n));
     tree_q = take(32 * ((8 * n) - 1));
-    f1 = take(4 * 4 * 2 * n);
-    f2 = take(4 * 4 * 4 * n);
-    for (uint32_t j = 1; j <= L.n_line + 1; j++) line.push_back(take(16ull << (p.log_n + 2 - j)));
+    f1 = take(4 * 4 * cs(2 * n));
+    f2 = take(4 * 4 * cs(4 * n));
+    for (uint32_t j = 1; j <= L.n_line + 1; j++) line.push_back(take(4 * 4 * cs(1ull << (p.log_n + 2 - j))));
     for (uint32_t j = 1; j <= L.n_line; j++) tree_l.push_back(take(32 * ((2ull << (p.log_n + 2 - j)) - 1)));
     eap = take(std::max(eval_at_point_columns_scratch_bytes(p.n_cols, p.log_n),
                         eval_at_point_columns_scratch_bytes(4, p.log_n + 1)));
@@ -370,10 +377,10 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
 
   // device pointer tables: coeffs | lde | comp | comp_lde
   std::vector<const uint32_t*> t;
-  for (uint32_t i = 0; i < C; i++) t.push_back(coeffs + i * n);
-  for (uint32_t i = 0; i < C; i++) t.push_back(lde + i * 2 * n);
-  for (uint32_t k = 0; k < 4; k++) t.push_back(comp + k * 2 * n);
-  for (uint32_t k = 0; k < 4; k++) t.push_back(comp_lde + k * 4 * n);
+  for (uint32_t i = 0; i < C; i++) t.push_back(coeffs + i * cs(n));
+  for (uint32_t i = 0; i < C; i++) t.push_back(lde + i * cs(2 * n));
+  for (uint32_t k = 0; k < 4; k++) t.push_back(comp + k * cs(2 * n));
+  for (uint32_t k = 0; k < 4; k++) t.push_back(comp_lde + k * cs(4 * n));
   // one device region for every table of the job: pointer tables, the Merkle plans'
   // tables and the gather table, filled as one host image and uploaded once
   const uint32_t NL = L.n_line;
@@ -414,7 +421,7 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
     const uint32_t m = ln + 2 - j;
     layers_l.push_back(tree_layers(base + A.tree_l[j - 1], m));
     const uint32_t* cols[4];
-    for (int k = 0; k < 4; k++) cols[k] = line[j - 1] + ((uint64_t)k << m);
+    for (int k = 0; k < 4; k++) cols[k] = line[j - 1] + k * cs(1ull << m);
     const uint32_t l4[4] = {m, m, m, m};
     tree_l.push_back(std::make_unique<merkle::MerklePlan>(4, cols, l4, layers_l.back().data(), m,
                                                           dtab + o_tree[1 + j], image.data() + o_tree[1 + j]));
@@ -445,23 +452,23 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
     const uint32_t* last = line[L.n_line];
     const uint32_t nl = 1u << p.last_log;
     for (uint32_t v = 0; v < nl; v++)
-      for (uint32_t k = 0; k < 4; k++) fixed(last + (uint64_t)k * nl + v, L.last + 4 * v + k, 1);
+      for (uint32_t k = 0; k < 4; k++) fixed(last + k * cs(nl) + v, L.last + 4 * v + k, 1);
   }
   fixed(&jb->nonce, L.nonce, 2);
   fixed(jb->queries, L.queries, p.n_queries);
   for (uint32_t q = 0; q < p.n_queries; q++) {
     const uint64_t qb = L.per_query + q * L.query_words;
     for (uint32_t s = 0; s < 2; s++)
-      for (uint32_t i = 0; i < C; i++) indexed(lde + i * 2 * n, qb + L.q_trace_vals + s * C + i, 1, q, 1, s);
+      for (uint32_t i = 0; i < C; i++) indexed(lde + i * cs(2 * n), qb + L.q_trace_vals + s * C + i, 1, q, 1, s);
     path(layers_t, ln + 1, qb + L.q_trace_path, q, 1);
     for (uint32_t s = 0; s < 2; s++)
-      for (uint32_t k = 0; k < 4; k++) indexed(comp_lde + k * 4 * n, qb + L.q_comp_vals + 4 * s + k, 1, q, 0, s);
+      for (uint32_t k = 0; k < 4; k++) indexed(comp_lde + k * cs(4 * n), qb + L.q_comp_vals + 4 * s + k, 1, q, 0, s);
     path(layers_q, ln + 2, qb + L.q_comp_path, q, 0);
     for (uint32_t j = 1; j <= L.n_line; j++) {
       const uint32_t m = ln + 2 - j;
       for (uint32_t s = 0; s < 2; s++)
         for (uint32_t k = 0; k < 4; k++)
-          indexed(line[j - 1] + ((uint64_t)k << m), qb + L.q_line[j - 1] + 4 * s + k, 1, q, j, s);
+          indexed(line[j - 1] + k * cs(1ull << m), qb + L.q_line[j - 1] + 4 * s + k, 1, q, j, s);
       path(layers_l[j - 1], m, qb + L.q_line[j - 1] + 8, q, j);
     }
   }
@@ -501,7 +508,7 @@ void Job::enqueue(uint64_t seed) {
 
   qm31_powers_async(&jb->alpha, C - 2, powers);
   vanishing_inverse_async(dinv, ln + 1, ln);
-  uint32_t* q4[4] = {comp, comp + 2 * n, comp + 4 * n, comp + 6 * n};
+  uint32_t* q4[4] = {comp, comp + cs(2 * n), comp + 2 * cs(2 * n), comp + 3 * cs(2 * n)};
   wide_fib_quotients_async(q4, t_lde, powers, dinv, (uint32_t)(2 * n), C);
   fft::lde_same_size(t_comp, (uint32_t* const*)t_comp, (uint32_t* const*)t_comp_lde, 4, ln + 1, 1, tw, itw, tree_len,
                      fft::lde_variant_for(ln + 1, 1, 4));
@@ -512,8 +519,8 @@ void Job::enqueue(uint64_t seed) {
   eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch);
   timed("channel_samples", [&] { k_mix_samples_draw<<<1, TREE_NT, 0, stream>>>(jb, samples, C + 4); });
 
-  uint32_t* f1c[4] = {f1, f1 + 2 * n, f1 + 4 * n, f1 + 6 * n};
-  uint32_t* f2c[4] = {f2, f2 + 4 * n, f2 + 8 * n, f2 + 12 * n};
+  uint32_t* f1c[4] = {f1, f1 + cs(2 * n), f1 + 2 * cs(2 * n), f1 + 3 * cs(2 * n)};
+  uint32_t* f2c[4] = {f2, f2 + cs(4 * n), f2 + 2 * cs(4 * n), f2 + 3 * cs(4 * n)};
   accumulate_quotients_single_point_async(half_initial(ln + 1), half_step(ln + 1), ln + 1, t_lde, C, jb->z, samples,
                                           &jb->beta, quot_scratch, f1c);
   accumulate_quotients_single_point_async(half_initial(ln + 2), half_step(ln + 2), ln + 2, t_comp_lde, 4, jb->z,
@@ -521,7 +528,7 @@ void Job::enqueue(uint64_t seed) {
 
   auto coords = [&](uint32_t* c, uint32_t m) {
     std::array<uint32_t*, 4> r;
-    for (int k = 0; k < 4; k++) r[k] = c + ((uint64_t)k << m);
+    for (int k = 0; k < 4; k++) r[k] = c + k * cs(1ull << m);
     return r;
   };
   auto l1 = coords(line[0], ln + 1);
@@ -540,7 +547,7 @@ void Job::enqueue(uint64_t seed) {
   {
     const uint32_t nl = 1u << p.last_log;
     const uint32_t* last = line[L.n_line];
-    Soa4c s{{last, last + nl, last + 2 * nl, last + 3 * nl}};
+    Soa4c s{{last, last + cs(nl), last + 2 * cs(nl), last + 3 * cs(nl)}};
     timed("channel_last_layer", [&] { k_mix_last<<<1, TREE_NT, 0, stream>>>(jb, s, nl); });
   }
   grind_async(jb->ch.digest, p.pow_bits, &jb->nonce);

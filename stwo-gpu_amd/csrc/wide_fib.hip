@@ -18,6 +18,8 @@ namespace wfib {
 
 __device__ __forceinline__ uint64_t fold64(uint64_t s) { return (s & P) + (s >> 31); }
 
+constexpr int WG = 6;  // columns per load group (a multiple of the fold period 3)
+
 __global__ __launch_bounds__(256) void quotients_kernel(uint32_t n_rows, uint32_t n_cols,
                                                         const uint32_t* const* __restrict__ trace,
                                                         const uint4* __restrict__ powers,
@@ -27,22 +29,36 @@ __global__ __launch_bounds__(256) void quotients_kernel(uint32_t n_rows, uint32_
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  uint32_t a = trace[0][r], b = trace[1][r];
-  uint32_t sa = mul(a, a), sb = mul(b, b);
-  for (uint32_t i = 0; i + 2 < n_cols; i++) {
-    uint32_t c = trace[i + 2][r];
-    uint64_t cst = sub(add(sa, sb), c);
-    uint4 pw = powers[n_cols - 3 - i];
-    s0 += pw.x * cst;
-    s1 += pw.y * cst;
-    s2 += pw.z * cst;
-    s3 += pw.w * cst;
-    s0 = fold64(s0);
-    s1 = fold64(s1);
-    s2 = fold64(s2);
-    s3 = fold64(s3);
-    sa = sb;
-    sb = mul(c, c);
+  uint32_t sa = sqr(trace[0][r]), sb = sqr(trace[1][r]);
+  // constraint i reads column i + 2; columns are loaded one group ahead of the arithmetic
+  const uint32_t n_cons = n_cols - 2;
+  uint32_t cv[WG], cn[WG];
+#pragma unroll
+  for (int g = 0; g < WG; g++) cv[g] = (uint32_t)g < n_cons ? trace[2 + g][r] : 0u;
+  for (uint32_t i = 0; i < n_cons; i += WG) {
+#pragma unroll
+    for (int g = 0; g < WG; g++) cn[g] = i + WG + g < n_cons ? trace[2 + i + WG + g][r] : 0u;
+#pragma unroll
+    for (int g = 0; g < WG; g++) {
+      if (i + g >= n_cons) break;
+      const uint32_t c = cv[g];
+      const uint64_t cst = sub(add(sa, sb), c);
+      const uint4 pw = powers[n_cons - 1 - (i + g)];
+      s0 += pw.x * cst;
+      s1 += pw.y * cst;
+      s2 += pw.z * cst;
+      s3 += pw.w * cst;
+      if (g % 3 == 2) {  // at most three products between folds
+        s0 = fold64(s0);
+        s1 = fold64(s1);
+        s2 = fold64(s2);
+        s3 = fold64(s3);
+      }
+      sa = sb;
+      sb = sqr(c);
+    }
+#pragma unroll
+    for (int g = 0; g < WG; g++) cv[g] = cn[g];
   }
   uint32_t d = dinv[r];
   q0[r] = mul(reduce63(s0), d);
