@@ -70,8 +70,8 @@ __device__ __forceinline__ void compress(uint32_t (&h)[8], const uint32_t (&m)[1
 }
 
 // Absorb the columns of one layer at `row` (16-column chunks, zero-padded remainder).
-__device__ __forceinline__ void absorb_columns(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
-                                               uint32_t row) {
+__device__ __forceinline__ void absorb_columns_simple(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
+                                                      uint32_t row) {
   uint32_t c = 0;
   for (; c + 16 <= n_cols; c += 16) {
     uint32_t m[16];
@@ -86,7 +86,6 @@ __device__ __forceinline__ void absorb_columns(uint32_t (&h)[8], const uint32_t*
     compress(h, m);
   }
 }
-
 
 __device__ __forceinline__ void store_hash(Hash* out, uint32_t i, const uint32_t (&h)[8]) {
   uint4* o = reinterpret_cast<uint4*>(out + i);
@@ -113,10 +112,115 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
     load_children(prev, i, m);
     compress(h, m);
   }
-  absorb_columns(h, cols, n_cols, i);
+  absorb_columns_simple(h, cols, n_cols, i);
   store_hash(out, i, h);
 }
 
+
+// ---- one compression on the four lanes of a quad ------------------------------------
+// Narrow layers (few nodes) leave most lanes idle and their hash chains are latency
+// bound, so there a node is hashed by a quad: lane q keeps state words v[q], v[4+q],
+// v[8+q], v[12+q] and computes the column-step G on them; for the diagonal step b, c, d
+// are rotated by 1, 2, 3 lanes inside the quad (DPP quad permutes) and back.  The lane
+// reads the 40 message words it needs (two per G, ten rounds) from the node's 16-word
+// message in LDS before the rounds start.
+constexpr uint8_t SIGMA[10][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
+    {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4}, {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
+    {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13}, {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
+    {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11}, {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
+    {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0}};
+// message index of word k (0..3: column x, column y, diagonal x, diagonal y) of round r,
+// for the four lanes of a quad packed in bytes
+constexpr uint32_t quad_index(int r, int k) {
+  uint32_t v = 0;
+  for (int q = 0; q < 4; q++) v |= (uint32_t)SIGMA[r][(k < 2 ? 0 : 8) + 2 * q + (k & 1)] << (8 * q);
+  return v;
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t qperm(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+}
+constexpr int QROT1 = 0x39, QROT2 = 0x4E, QROT3 = 0x93;  // lane q reads lane q+1, q+2, q+3 (mod 4)
+
+#define QG(a, b, c, d, x, y) \
+  a = a + b + (x);           \
+  d = rotr(d ^ a, 16);       \
+  c = c + d;                 \
+  b = rotr(b ^ c, 12);       \
+  a = a + b + (y);           \
+  d = rotr(d ^ a, 8);        \
+  c = c + d;                 \
+  b = rotr(b ^ c, 7);
+
+// (h_lo, h_hi) = (h[q], h[4+q]) of the node's state; msg: the node's 16 message words in LDS.
+__device__ __forceinline__ void quad_compress(uint32_t& h_lo, uint32_t& h_hi, const uint32_t* msg, uint32_t q) {
+  const uint32_t sh = 8 * q;
+  uint32_t m[40];
+#pragma unroll
+  for (int r = 0; r < 10; r++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[4 * r + k] = msg[(quad_index(r, k) >> sh) & 15];
+  constexpr uint32_t IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                              0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+  uint32_t a = h_lo, b = h_hi;
+  uint32_t c = q == 0 ? IV[0] : q == 1 ? IV[1] : q == 2 ? IV[2] : IV[3];
+  uint32_t d = q == 0 ? IV[4] : q == 1 ? IV[5] : q == 2 ? IV[6] : IV[7];
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    QG(a, b, c, d, m[4 * r], m[4 * r + 1])
+    b = qperm<QROT1>(b);
+    c = qperm<QROT2>(c);
+    d = qperm<QROT3>(d);
+    QG(a, b, c, d, m[4 * r + 2], m[4 * r + 3])
+    b = qperm<QROT3>(b);
+    c = qperm<QROT2>(c);
+    d = qperm<QROT1>(d);
+  }
+  h_lo ^= a ^ c;
+  h_hi ^= b ^ d;
+}
+#undef QG
+// Narrow layer, one quad per node (4 x size lanes): children compress, then the 16-column
+// chunks, each staged through the quad's 16 LDS words.
+__global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t n_cols, const uint32_t* const* cols,
+                                                         const Hash* prev, Hash* out) {
+  __shared__ uint32_t msg[64][16];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = t >> 2, q = t & 3, lq = threadIdx.x >> 2;
+  if (i >= size) return;  // whole quads leave together
+  uint32_t* mb = msg[lq];
+  uint32_t h_lo = 0, h_hi = 0;
+  if (prev) {
+    const uint4 w = reinterpret_cast<const uint4*>(prev + 2 * (size_t)i)[q];
+    mb[4 * q] = w.x;
+    mb[4 * q + 1] = w.y;
+    mb[4 * q + 2] = w.z;
+    mb[4 * q + 3] = w.w;
+    __builtin_amdgcn_wave_barrier();
+    quad_compress(h_lo, h_hi, mb, q);
+  }
+  // column chunks: the next chunk's loads are in flight while the current one is hashed
+  uint32_t nx[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) nx[k] = 4 * q + k < n_cols ? cols[4 * q + k][i] : 0u;
+  for (uint32_t c = 0; c < n_cols; c += 16) {
+    uint32_t cur[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      cur[k] = nx[k];
+      const uint32_t col = c + 16 + 4 * q + k;
+      nx[k] = col < n_cols ? cols[col][i] : 0u;
+    }
+    __builtin_amdgcn_wave_barrier();  // the quad's reads of the previous message are done
+#pragma unroll
+    for (int k = 0; k < 4; k++) mb[4 * q + k] = cur[k];
+    __builtin_amdgcn_wave_barrier();
+    quad_compress(h_lo, h_hi, mb, q);
+  }
+  out[i].s[q] = h_lo;
+  out[i].s[4 + q] = h_hi;
+}
 
 // Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b reads
 // 2^(w_log+1) children hashes of layer top+1 (prev), produces 2^w_log nodes of layer
@@ -129,6 +233,7 @@ constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
 __global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
                                                       Hash* const* outs, const LayerCols* inj) {
   extern __shared__ Hash s[];  // 2^w_log hashes (dynamic: shallow launches fit more blocks per CU)
+  __shared__ uint32_t qmsg[64][16];  // quad mode: staging of injected column chunks
   const uint32_t nodes0 = 1u << w_log;
   const uint32_t base0 = blockIdx.x << w_log;
   for (uint32_t d = 0; d < depth; d++) {
@@ -136,33 +241,61 @@ __global__ __launch_bounds__(256) void subtree_kernel(const Hash* prev, uint32_t
     const uint32_t cnt = nodes0 >> d;
     const uint32_t base = base0 >> d;
     const LayerCols lc = inj[layer];
-    uint32_t h_keep[4][8];
-    uint32_t n_mine = 0;
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x, n_mine++) {
-      uint32_t m[16];
-      if (d == 0) {
-        load_children(prev, base + i, m);
-      } else {
+    if (d > 0 && cnt <= blockDim.x / 4) {
+      // narrow level: one quad per node, children read from s in place
+      const uint32_t i = threadIdx.x >> 2, q = threadIdx.x & 3;
+      uint32_t h_lo = 0, h_hi = 0;
+      if (i < cnt) {
+        quad_compress(h_lo, h_hi, s[2 * i].s, q);
+        uint32_t* mb = qmsg[i];
+        for (uint32_t c = 0; c < lc.n_cols; c += 16) {
+          __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-          m[w] = s[2 * i].s[w];
-          m[8 + w] = s[2 * i + 1].s[w];
+          for (int k = 0; k < 4; k++) {
+            const uint32_t col = c + 4 * q + k;
+            mb[4 * q + k] = col < lc.n_cols ? lc.cols[col][base + i] : 0u;
+          }
+          __builtin_amdgcn_wave_barrier();
+          quad_compress(h_lo, h_hi, mb, q);
         }
+        outs[layer][base + i].s[q] = h_lo;
+        outs[layer][base + i].s[4 + q] = h_hi;
       }
+      __syncthreads();  // all reads of s for this layer are done
+      if (i < cnt) {
+        s[i].s[q] = h_lo;
+        s[i].s[4 + q] = h_hi;
+      }
+      __syncthreads();
+      continue;
+    }
+    // one node per lane per round; a round's results are written back after a barrier, and
+    // later rounds read children at indices above everything written so far
+    for (uint32_t r0 = 0; r0 < cnt; r0 += blockDim.x) {
+      const uint32_t i = r0 + threadIdx.x;
       uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      compress(h, m);
-      absorb_columns(h, lc.cols, lc.n_cols, base + i);
-      store_hash(outs[layer], base + i, h);
+      if (i < cnt) {
+        uint32_t m[16];
+        if (d == 0) {
+          load_children(prev, base + i, m);
+        } else {
 #pragma unroll
-      for (int w = 0; w < 8; w++) h_keep[n_mine & 3][w] = h[w];
-    }
-    __syncthreads();  // all reads of s for this layer are done
-    n_mine = 0;
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x, n_mine++) {
+          for (int w = 0; w < 8; w++) {
+            m[w] = s[2 * i].s[w];
+            m[8 + w] = s[2 * i + 1].s[w];
+          }
+        }
+        compress(h, m);
+        absorb_columns_simple(h, lc.cols, lc.n_cols, base + i);
+        store_hash(outs[layer], base + i, h);
+      }
+      __syncthreads();  // this round's reads of s are done
+      if (i < cnt) {
 #pragma unroll
-      for (int w = 0; w < 8; w++) s[i].s[w] = h_keep[n_mine & 3][w];
+        for (int w = 0; w < 8; w++) s[i].s[w] = h[w];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -172,6 +305,7 @@ static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 // compressions); it gets its own launch with one lane per node, so all of its nodes
 // run in parallel.  Runs of layers with few columns are built by subtree launches.
 constexpr uint32_t HEAVY_CHUNKS = 2;
+constexpr uint32_t QUAD_LAYER_MAX = 14;  // layers of at most 2^14 nodes are hashed by quads
 constexpr int32_t WIDE_LAYER = 16;
 
 size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log) {
@@ -256,7 +390,12 @@ void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
       const uint32_t k = s.top;
       const Hash* prev = k == max_log ? nullptr : outs[k + 1];
       timed(k == max_log ? "merkle_leaf" : "merkle_layer", [&] {
-        layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev, outs[k]);
+        if (k <= QUAD_LAYER_MAX)
+          quad_layer_kernel<<<blocks_for(4ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev,
+                                                                          outs[k]);
+        else
+          layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev,
+                                                                     outs[k]);
       });
     } else {
       // at least 256 nodes of layer top per block (or the whole layer when it is smaller)
