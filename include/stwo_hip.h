@@ -270,6 +270,14 @@ int stwo_hip_commit_plan_destroy(StwoCommitPlan *plan);
 int stwo_hip_lde_schedule(uint32_t log_size, uint32_t log_blowup, uint64_t n_cols, uint32_t *contiguous_layers,
                           int *fused);
 
+/* Batched OODS evaluation (PolyOps::eval_at_point, stwo_gpu_backend/src/poly.rs:84-94, for
+ * many polynomials at once): values[c] = value at (point_x, point_y) of the circle polynomial
+ * with coefficients columns[c] (2^log_size each; HOST array of device columns).  Columns of
+ * log >= 12 are evaluated as one matrix-vector product against a factored basis table;
+ * 16-byte aligned columns are required there. */
+int stwo_hip_eval_at_point_columns(const uint32_t *const *columns, uint32_t n_cols, uint32_t log_size,
+                                   StwoQM31 point_x, StwoQM31 point_y, StwoQM31 *values);
+
 /* Proof of work (GrindOps<Blake2sChannel>::grind, which the reference delegates to the CPU
  * SimdBackend, stwo_gpu_backend/src/backend.rs:14-18): the smallest nonce whose
  * BLAKE2s-256(digest || lo32(nonce) || hi32(nonce)) starts (as a little-endian 64-bit

@@ -99,31 +99,97 @@ __global__ __launch_bounds__(NT) void first_pass_cols(const uint32_t* const* __r
   if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
 }
 
-// Columns of log >= 12: block (b, col) folds coefficients [b 2^12, (b+1) 2^12) of column
-// col; each lane folds 16 consecutive coefficients in registers (bits 0..3: factors
-// f[L-1] .. f[L-4]) from four 16-byte loads, then the 256 lane values are folded in LDS.
-__global__ __launch_bounds__(NT) void first_pass_cols16(const uint32_t* const* __restrict__ cols, uint32_t L,
-                                                        const Factors* __restrict__ F, qm31* partial) {
-  __shared__ qm31 s[NT];
-  const uint4* c = reinterpret_cast<const uint4*>(cols[blockIdx.y] + ((size_t)blockIdx.x << 12) + 16 * threadIdx.x);
-  const uint4 q0 = c[0], q1 = c[1], q2 = c[2], q3 = c[3];
-  const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                          q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-  const qm31 f0 = F->f[L - 1], f1 = F->f[L - 2], f2 = F->f[L - 3], f3 = F->f[L - 4];
-  qm31 a[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    a[j] = qscale(f0, v[2 * j + 1]);
-    a[j].x.a = add(a[j].x.a, v[2 * j]);
+// ---- columns of log >= 12 as a matrix-vector product -------------------------------
+// The value at the point is sum_i c_i B_i with B_i the product of the factors selected by
+// the bits of i.  Split i = 4096 h + j: B_i = Lo[j] Hi[h], Lo over the low 12 bits and Hi
+// over the rest (two small tables per point).  A block takes 4096 coefficients (one h) of
+// a group of columns; a lane multiplies its 16 coefficients by its 16 Lo values held in
+// registers (M31 x QM31 products accumulated in 64 bits), the block sums the lanes, and
+// the sum times Hi[h] is the block's partial; partials are summed per column afterwards.
+// Every coefficient is read once; about 12 VALU instructions per coefficient.
+constexpr uint32_t LO_BITS = 12, GEMV_COLS = 8;
+
+__global__ __launch_bounds__(256) void basis_tables_kernel(const Factors* __restrict__ F, uint32_t L, qm31* lo,
+                                                           qm31* hi) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_lo = 1u << LO_BITS, n_hi = 1u << (L - LO_BITS);
+  if (t < n_lo) {
+    qm31 v = qone();
+    for (uint32_t b = 0; b < LO_BITS; b++)
+      if ((t >> b) & 1) v = qmul(v, F->f[L - 1 - b]);
+    lo[t] = v;
+  } else if (t < n_lo + n_hi) {
+    const uint32_t h = t - n_lo;
+    qm31 v = qone();
+    for (uint32_t b = 0; b < L - LO_BITS; b++)
+      if ((h >> b) & 1) v = qmul(v, F->f[L - 1 - LO_BITS - b]);
+    hi[h] = v;
   }
+}
+
+__device__ __forceinline__ uint32_t addm(uint32_t a, uint32_t b) { return add(a, b); }
+__device__ __forceinline__ qm31 wave_sum(qm31 v) {
 #pragma unroll
-  for (int j = 0; j < 4; j++) a[j] = qadd(a[2 * j], qmul(a[2 * j + 1], f1));
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x.a = addm(v.x.a, (uint32_t)__shfl_xor((int)v.x.a, o));
+    v.x.b = addm(v.x.b, (uint32_t)__shfl_xor((int)v.x.b, o));
+    v.y.a = addm(v.y.a, (uint32_t)__shfl_xor((int)v.y.a, o));
+    v.y.b = addm(v.y.b, (uint32_t)__shfl_xor((int)v.y.b, o));
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                   const qm31* __restrict__ lo, const qm31* __restrict__ hi,
+                                                   qm31* __restrict__ partial) {
+  __shared__ qm31 wsum[GEMV_COLS][4];
+  const uint32_t h = blockIdx.x, c0 = blockIdx.y * GEMV_COLS, t = threadIdx.x;
+  const uint32_t n_hi = gridDim.x;
+  qm31 l[16];
 #pragma unroll
-  for (int j = 0; j < 2; j++) a[j] = qadd(a[2 * j], qmul(a[2 * j + 1], f2));
-  s[threadIdx.x] = qadd(a[0], qmul(a[1], f3));
+  for (int j = 0; j < 16; j++) l[j] = lo[16 * t + j];
+  const size_t off = ((size_t)h << LO_BITS) + 16 * t;
+  for (uint32_t g = 0; g < GEMV_COLS && c0 + g < n_cols; g++) {
+    const uint4* p = reinterpret_cast<const uint4*>(cols[c0 + g] + off);
+    const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      s0 += (uint64_t)l[j].x.a * v[j];
+      s1 += (uint64_t)l[j].x.b * v[j];
+      s2 += (uint64_t)l[j].y.a * v[j];
+      s3 += (uint64_t)l[j].y.b * v[j];
+      if (j % 3 == 2) {  // at most three products between folds
+        s0 = (s0 & P) + (s0 >> 31);
+        s1 = (s1 & P) + (s1 >> 31);
+        s2 = (s2 & P) + (s2 >> 31);
+        s3 = (s3 & P) + (s3 >> 31);
+      }
+    }
+    qm31 r = {{reduce63(s0), reduce63(s1)}, {reduce63(s2), reduce63(s3)}};
+    r = wave_sum(r);
+    if ((t & 63) == 0) wsum[g][t >> 6] = r;
+  }
   __syncthreads();
-  tree_reduce(s, NT, *F, (int)L - 5);
-  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+  if (t < GEMV_COLS && c0 + t < n_cols) {
+    qm31 r = qadd(qadd(wsum[t][0], wsum[t][1]), qadd(wsum[t][2], wsum[t][3]));
+    partial[(size_t)(c0 + t) * n_hi + h] = qmul(r, hi[h]);
+  }
+}
+
+// out[c] = sum of the n_hi partials of column c (one block per column)
+__global__ __launch_bounds__(256) void sum_partials_kernel(const qm31* __restrict__ partial, uint32_t n_hi,
+                                                           qm31* __restrict__ out) {
+  __shared__ qm31 ws[4];
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  qm31 r = qzero();
+  for (uint32_t h = t; h < n_hi; h += 256) r = qadd(r, partial[(size_t)c * n_hi + h]);
+  r = wave_sum(r);
+  if ((t & 63) == 0) ws[t >> 6] = r;
+  __syncthreads();
+  if (t == 0) out[c] = qadd(qadd(ws[0], ws[1]), qadd(ws[2], ws[3]));
 }
 
 __global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g, const Factors* __restrict__ F,
@@ -139,7 +205,9 @@ __global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g
 }  // namespace eap
 
 size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L) {
-  const uint32_t cb = L >= 12 ? 12 : std::min<uint32_t>(L, eap::CB);
+  if (L >= eap::LO_BITS)  // factors | Lo | Hi | partials
+    return 512 + sizeof(qm31) * ((1ull << eap::LO_BITS) + (1ull << (L - eap::LO_BITS)) * (1 + (size_t)n_cols));
+  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
   return 512 + 2 * sizeof(qm31) * (size_t)n_cols * (1ull << (L - cb));
 }
 
@@ -148,17 +216,29 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
                                  qm31* d_out, void* d_scratch) {
   require(L >= 1 && L <= 31 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
   eap::Factors* F = (eap::Factors*)d_scratch;
-  const uint32_t cb = L >= 12 ? 12 : std::min<uint32_t>(L, eap::CB);
+  if (L >= eap::LO_BITS) {
+    const uint32_t n_hi = 1u << (L - eap::LO_BITS);
+    qm31* lo = (qm31*)((char*)d_scratch + 512);
+    qm31* hi = lo + (1u << eap::LO_BITS);
+    qm31* partial = hi + n_hi;
+    timed("eval_at_point", [&] {
+      eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
+      eap::basis_tables_kernel<<<ceil_div((1u << eap::LO_BITS) + n_hi, 256), 256, 0, ctx().cur>>>(F, L, lo, hi);
+      eap::gemv_kernel<<<dim3(n_hi, ceil_div(n_cols, eap::GEMV_COLS)), 256, 0, ctx().cur>>>(d_cols, n_cols, lo, hi,
+                                                                                              partial);
+      eap::sum_partials_kernel<<<n_cols, 256, 0, ctx().cur>>>(partial, n_hi, d_out);
+    });
+    return;
+  }
+  // small columns (log < 12): fold in LDS, block results reduced by later passes
+  const uint32_t cb = std::min<uint32_t>(L, eap::CB);
   uint32_t blocks = 1u << (L - cb);
   qm31* pa = (qm31*)((char*)d_scratch + 512);
   qm31* pb = pa + (size_t)n_cols * blocks;
   timed("eval_at_point", [&] {
     eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
-    if (cb == 12)
-      eap::first_pass_cols16<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, F, blocks == 1 ? d_out : pa);
-    else
-      eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
-                                                                             blocks == 1 ? d_out : pa);
+    eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
+                                                                           blocks == 1 ? d_out : pa);
   });
   int fi = (int)(L - cb) - 1;
   while (blocks > 1) {
@@ -226,4 +306,23 @@ extern "C" StwoQM31 eval_at_point(const uint32_t* coeffs, uint32_t coeffs_size, 
     out = StwoQM31{r.x.a, r.x.b, r.y.a, r.y.b};
   });
   return out;
+}
+
+extern "C" int stwo_hip_eval_at_point_columns(const uint32_t* const* columns, uint32_t n_cols, uint32_t log_size,
+                                              StwoQM31 point_x, StwoQM31 point_y, StwoQM31* values) {
+  return guarded([&] {
+    require(columns != nullptr && values != nullptr && n_cols >= 1 && log_size >= 1 && log_size <= 30,
+            "eval_at_point_columns: bad arguments");
+    const qm31 pt[2] = {qm31{{point_x.a, point_x.b}, {point_x.c, point_x.d}},
+                        qm31{{point_y.a, point_y.b}, {point_y.c, point_y.d}}};
+    DevBuf table = upload(columns, (size_t)n_cols * sizeof(void*));
+    DevBuf dpt = upload(pt, sizeof(pt));
+    DevBuf out(sizeof(qm31) * n_cols);
+    DevBuf scratch(eval_at_point_columns_scratch_bytes(n_cols, log_size));
+    eval_at_point_columns_async(table.as<const uint32_t*>(), n_cols, log_size, dpt.as<qm31>(), out.as<qm31>(),
+                                scratch.p);
+    hip_check(hipMemcpyAsync(values, out.p, sizeof(qm31) * n_cols, hipMemcpyDeviceToHost, ctx().cur),
+              "hipMemcpyAsync D2H");
+    sync();
+  });
 }

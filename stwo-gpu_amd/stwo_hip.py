@@ -124,6 +124,7 @@ def _declare(L):
         "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
         "stwo_hip_lde_schedule": (C.c_int, [u32, u32, u64, u32p, C.POINTER(C.c_int)]),
         "stwo_hip_grind": (C.c_int, [u32p, u32, C.POINTER(u64)]),
+        "stwo_hip_eval_at_point_columns": (C.c_int, [pp, u32, u32, QM31, QM31, vp]),
         "stwo_hip_wide_fib_trace": (C.c_int, [pp, u32, u32, u64]),
         "stwo_hip_vanishing_inverse": (C.c_int, [vp, u32, u32]),
         "stwo_hip_wide_fib_proof_size": (C.c_int, [C.POINTER(WideFibParams), C.POINTER(u64), C.POINTER(u64)]),
@@ -703,6 +704,15 @@ class CommitPlan:
 
 # ----------------------------------------------------------------- proofs
 NOT_READY = 5
+
+
+def eval_at_point_columns(cols, px, py):
+    """Values of the polynomials with coefficients `cols` (same size) at the QM31 point (px, py)."""
+    log = cols[0].size.bit_length() - 1
+    out = (QM31 * len(cols))()
+    _c(lib().stwo_hip_eval_at_point_columns(_ptr_array([c.ptr for c in cols]), len(cols), log, qm31(px), qm31(py),
+                                            C.cast(out, vp)))
+    return [from_qm31(v) for v in out]
 
 
 def grind(digest, pow_bits):

@@ -501,3 +501,14 @@ def test_commit_plan_mixed_sizes(use_graph, autotune):
             assert (plan.layers[k].to_numpy().reshape(-1) == want[k]).all(), f"rep {rep} layer {k}"
         assert (plan.root() == want[0]).all()
     plan.close()
+
+
+# ------------------------------------------------------------------ batched OODS (poly.rs:84-94 over many columns)
+@pytest.mark.parametrize("log_n,n_cols", [(1, 3), (5, 7), (11, 9), (12, 8), (13, 17), (16, 5), (20, 2)])
+def test_eval_at_point_columns(log_n, n_cols):
+    rng = random.Random(90 + log_n)
+    px, py = _point(rng)
+    cols = [rand_m31(1 << log_n, 9000 + 17 * log_n + c) for c in range(n_cols)]
+    got = S.eval_at_point_columns([S.BaseFieldVec.from_numpy(c) for c in cols], px, py)
+    for c in range(n_cols):
+        assert tuple(got[c]) == O.eval_at_point(cols[c], px, py), f"column {c}"
