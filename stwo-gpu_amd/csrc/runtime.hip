@@ -69,6 +69,16 @@ std::unique_lock<std::recursive_mutex> lock() {
 }
 Context& ctx() { return g_ctx; }
 
+hipStream_t slot_stream(size_t i) {
+  Context& c = g_ctx;
+  while (c.slots.size() <= i) {
+    hipStream_t s;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    c.slots.push_back(s);
+  }
+  return c.slots[i];
+}
+
 static void resolve_timings() {
   Context& c = g_ctx;
   std::vector<TimedLaunch> later;

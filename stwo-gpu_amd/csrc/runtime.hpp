@@ -87,6 +87,10 @@ struct Context {
   hipStream_t cur = nullptr;     // where kernels are launched now (the library stream, or an
                                  // auxiliary stream while a plan fans independent work out)
   hipStream_t aux[N_AUX_STREAMS] = {};
+  // streams for concurrent jobs (proof server slots); they live as long as the process,
+  // since cached pool blocks, staging buffers and timing events keep referring to the
+  // stream their last user ran on
+  std::vector<hipStream_t> slots;
   hipEvent_t fork = nullptr, join[N_AUX_STREAMS] = {};
   Pool pool;
   std::map<TwiddleKey, TwiddleEntry> twiddles;
@@ -103,6 +107,7 @@ struct Context {
 std::unique_lock<std::recursive_mutex> lock();
 Context& ctx();
 void sync();  // synchronise the library stream (resolves pending kernel timings)
+hipStream_t slot_stream(size_t i);  // the i-th job slot stream, created on first use
 
 // Device buffer from the pool, returned to the pool on scope exit.
 struct DevBuf {

@@ -205,7 +205,7 @@ extern "C" int stwo_hip_server_create(StwoProofServer** out, uint64_t hbm_budget
     s.max_inflight = max_inflight;
     s.streams.resize(max_inflight);
     s.slot_busy.assign(max_inflight, false);
-    for (auto& st : s.streams) hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    for (uint32_t i = 0; i < max_inflight; i++) s.streams[i] = slot_stream(i);
     const int dev = ctx().device;
     s.worker = std::thread([&s, dev] {
       (void)hipSetDevice(dev);
@@ -270,8 +270,6 @@ extern "C" int stwo_hip_server_destroy(StwoProofServer* h) {
     h->s.cv_work.notify_all();
   }
   if (h->s.worker.joinable()) h->s.worker.join();
-  return guarded([&] {
-    for (auto& st : h->s.streams) (void)hipStreamDestroy(st);
-    delete h;
-  });
+  // the slot streams belong to the library context and stay alive
+  return guarded([&] { delete h; });
 }

@@ -90,6 +90,23 @@ def test_server_admits_by_memory_budget():
         assert b[0] >= a[1] - 1e-6, "two jobs ran together although only one fits the budget"
 
 
+def test_servers_back_to_back():
+    """Servers opened and closed one after another in one process: buffers cached by the
+    pool and staging events from an earlier server's jobs are reused by the next one."""
+    shapes = [(32, 12), (64, 13), (128, 12), (32, 14)]
+    want = {i: S.wide_fib_prove(S.wide_fib_params(C, ln, 3, 6, 2), 50 + i) for i, (C, ln) in enumerate(shapes)}
+    for round_ in range(4):
+        srv = S.ProofServer(8 << 30, 4 + round_ % 2 * 4)
+        ids = {srv.submit(S.wide_fib_params(C, ln, 3, 6, 2), 50 + i): i for i, (C, ln) in enumerate(shapes)}
+        for _ in shapes:
+            r = srv.wait(5_000_000)
+            assert r is not None, "server stalled"
+            jid, st, proof, _ = r
+            assert st == 0, S.last_error()
+            assert (proof == want[ids[jid]]).all()
+        srv.close()
+
+
 @pytest.mark.parametrize("C,log_n,Q,pow_bits,last_log,seed", [(3, 4, 1, 0, 2, 1), (40, 6, 3, 6, 3, 2),
                                                               (33, 11, 3, 8, 2, 21), (64, 13, 2, 10, 4, 22)])
 def test_plain_path_matches_device_prover(C, log_n, Q, pow_bits, last_log, seed):
