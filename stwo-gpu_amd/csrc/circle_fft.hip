@@ -58,6 +58,18 @@ __device__ __forceinline__ SH_GLOBAL T* as_global(T* p) {
   return (SH_GLOBAL T*)p;
 }
 
+// element i of a column or level: the byte offset is formed in 32 bits (every index is
+// below 2^30), so the access is one global load/store with a scalar base and a 32-bit
+// vector offset instead of 64-bit address arithmetic per access
+template <class T>
+__device__ __forceinline__ SH_GLOBAL T& at(SH_GLOBAL T* base, uint32_t i) {
+  return *(SH_GLOBAL T*)((SH_GLOBAL char*)base + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ const SH_GLOBAL T& at(const SH_GLOBAL T* base, uint32_t i) {
+  return *(const SH_GLOBAL T*)((const SH_GLOBAL char*)base + i * (uint32_t)sizeof(T));
+}
+
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (I < N) {
@@ -90,7 +102,7 @@ struct Pass {
   const PassArgs& A;
   uint32_t* s;
   uint32_t tile, col_group, lob;
-  uint64_t block_base;  // index of tile position 0 (CONTIG / STRIDED)
+  uint32_t block_base;  // index of tile position 0 (CONTIG / STRIDED); columns hold < 2^31 values
   uint32_t hi_tile;     // tile's high index bits (twiddle block index)
   const SH_GLOBAL uint32_t* src;  // CONTIG / STRIDED: this block's column
   SH_GLOBAL uint32_t* dst;
@@ -114,14 +126,14 @@ struct Pass {
   // place the tile in high block `hi` (the bits above a + K) of its column
   __device__ __forceinline__ void set_hi(uint32_t hi) {
     hi_tile = hi;
-    block_base = ((uint64_t)hi_tile << (A.a + K)) | ((uint64_t)lob << T);
+    block_base = (hi_tile << (A.a + K)) | (lob << T);
   }
 
   // global index (within its column) of tile position p
-  __device__ __forceinline__ uint64_t index_of(uint32_t p) const {
+  __device__ __forceinline__ uint32_t index_of(uint32_t p) const {
     if constexpr (KIND == MULTI) return p & ((1u << K) - 1);
     else if constexpr (T == 0) return block_base + p;
-    else return block_base + ((uint64_t)(p >> T) << A.a) + (p & ((1u << T) - 1));
+    else return block_base + ((p >> T) << A.a) + (p & ((1u << T) - 1));
   }
   __device__ __forceinline__ uint32_t col_of(uint32_t p) const { return (col_group << HB) + (p >> K); }
 
@@ -133,10 +145,10 @@ struct Pass {
     if constexpr (KIND != STRIDED && BIT == 0) {
       // circle layer: [y, -y, -x, x] over pairs (x, y) of the first line level
       const uint32_t k2 = (h >> 2) << 1, sel = h & 3;
-      const uint32_t v = lvl[k2 + (sel < 2 ? 1 : 0)];
+      const uint32_t v = at(lvl, k2 + (sel < 2 ? 1 : 0));
       return (sel == 1 || sel == 2) ? neg(v) : v;
     } else {
-      return lvl[h];
+      return at(lvl, h);
     }
   }
   // line level used by layer bit b (global layer a + b >= 1); level 0 for the circle layer
@@ -160,19 +172,20 @@ struct Pass {
 #pragma unroll
         for (int r = 0; r < R; r++) v[gi * R + r] = c ? c[(pb + (r << LO)) & smask] : 0u;
       } else if constexpr (LO == 0 && T == 0 && W == 3) {
-        const uint64_t i0 = index_of(pb);
+        const uint32_t i0 = index_of(pb);
         if (A.src_log >= 3) {
-          const SH_GLOBAL uint4* q = reinterpret_cast<const SH_GLOBAL uint4*>(src + (i0 & smask));
-          uint4 x0 = q[0], x1 = q[1];
+          const SH_GLOBAL uint4* q = reinterpret_cast<const SH_GLOBAL uint4*>(src);
+          const uint32_t qi = (i0 & smask) >> 2;
+          uint4 x0 = at(q, qi), x1 = at(q, qi + 1);
           v[gi * 8 + 0] = x0.x; v[gi * 8 + 1] = x0.y; v[gi * 8 + 2] = x0.z; v[gi * 8 + 3] = x0.w;
           v[gi * 8 + 4] = x1.x; v[gi * 8 + 5] = x1.y; v[gi * 8 + 6] = x1.z; v[gi * 8 + 7] = x1.w;
         } else {
 #pragma unroll
-          for (int r = 0; r < 8; r++) v[gi * 8 + r] = src[(i0 + r) & smask];
+          for (int r = 0; r < 8; r++) v[gi * 8 + r] = at(src, (i0 + r) & smask);
         }
       } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) v[gi * R + r] = src[index_of(pb | (r << (T + LO))) & smask];
+        for (int r = 0; r < R; r++) v[gi * R + r] = at(src, index_of(pb | (r << (T + LO))) & smask);
       }
     }
   }
@@ -195,12 +208,13 @@ struct Pass {
           for (int r = 0; r < R; r++) c[(pb + (r << LO)) & ((1u << K) - 1)] = v[gi * R + r];
         }
       } else if constexpr (LO == 0 && T == 0 && W == 3) {
-        SH_GLOBAL uint4* q = reinterpret_cast<SH_GLOBAL uint4*>(dst + index_of(pb));
-        q[0] = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
-        q[1] = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
+        SH_GLOBAL uint4* q = reinterpret_cast<SH_GLOBAL uint4*>(dst);
+        const uint32_t qi = index_of(pb) >> 2;
+        at(q, qi) = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
+        at(q, qi + 1) = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
       } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) dst[index_of(pb | (r << (T + LO)))] = v[gi * R + r];
+        for (int r = 0; r < R; r++) at(dst, index_of(pb | (r << (T + LO)))) = v[gi * R + r];
       }
     }
   }
@@ -310,11 +324,20 @@ constexpr int BS_LDE = 14;
 // the separate strided passes measured faster (1.63 ms vs 1.73 ms for 256 columns of log
 // 20), so it is a variant for the autotuner rather than the default.
 constexpr int NT_S = 1024;  // threads per 2^14-element tile: 16 values per thread
+// Tile bits of the fused pass for K layers: 2^13-element tiles (8 values per thread, 32-word
+// runs) while K <= 8, so the kept coefficients and the forward copy fit in 64 VGPRs and two
+// blocks share a CU; 2^14 for K = 9.
 template <int K>
-__global__ __launch_bounds__(NT_S) void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
-  __shared__ uint32_t s[1 << BS_LDE];
-  using PI = Pass<true, BS_LDE, NT_S, K, STRIDED>;
-  using PF = Pass<false, BS_LDE, NT_S, K, STRIDED, false>;
+constexpr int lde_tile_bits() {
+  return K <= 8 ? 13 : BS_LDE;
+}
+template <int K>
+__global__ __launch_bounds__(NT_S) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 8 : 4)))
+void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
+  constexpr int BL = lde_tile_bits<K>();
+  __shared__ uint32_t s[1 << BL];
+  using PI = Pass<true, BL, NT_S, K, STRIDED>;
+  using PF = Pass<false, BL, NT_S, K, STRIDED, false>;
   constexpr int E = PI::E, NCH = PI::NCH;
   constexpr int LO_TOP = 3 * (NCH - 1), W_TOP = K - LO_TOP;
   PI pi(Ai, s);
@@ -590,7 +613,7 @@ void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint
     evaluate_same_size(d_lde, (const uint32_t* const*)d_coeffs, n_cols, n, m, tw, tree_len, kcf);
     return;
   }
-  const uint32_t k = n - 12, t = BS_LDE - k;
+  const uint32_t k = n - 12, t = (k <= 8 ? 13 : BS_LDE) - k;
   PassArgs Ai{};
   Ai.src = d_src;
   Ai.dst = d_coeffs;
