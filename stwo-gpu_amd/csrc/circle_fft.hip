@@ -25,6 +25,7 @@
 
 #include "fft.hpp"
 #include "field.hpp"
+#include "gmem.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -41,34 +42,10 @@ struct PassArgs {
   uint32_t a, k;     // this pass applies layers [a, a + k)
   uint32_t t, hb;    // tile bits: t low (contiguous) bits, hb high bits (or column bits)
   uint32_t multi;    // 1: the hb bits select a column (whole small columns per tile)
-  uint32_t scale;    // 0: none; else multiply outputs by it
+  uint32_t scale;    // 0: none; else multiply outputs by 2^scale (1 <= scale <= 30)
   uint32_t n_groups;     // column groups (grid x = n_groups * tiles_per_group)
   uint32_t lob_count_log;  // log2 of the number of low blocks per column (a - t)
 };
-
-// Column pointers come from device tables; tell the compiler they are global memory so
-// it emits global_* rather than flat_* accesses.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define SH_GLOBAL __attribute__((address_space(1)))
-#else
-#define SH_GLOBAL
-#endif
-template <class T>
-__device__ __forceinline__ SH_GLOBAL T* as_global(T* p) {
-  return (SH_GLOBAL T*)p;
-}
-
-// element i of a column or level: the byte offset is formed in 32 bits (every index is
-// below 2^30), so the access is one global load/store with a scalar base and a 32-bit
-// vector offset instead of 64-bit address arithmetic per access
-template <class T>
-__device__ __forceinline__ SH_GLOBAL T& at(SH_GLOBAL T* base, uint32_t i) {
-  return *(SH_GLOBAL T*)((SH_GLOBAL char*)base + i * (uint32_t)sizeof(T));
-}
-template <class T>
-__device__ __forceinline__ const SH_GLOBAL T& at(const SH_GLOBAL T* base, uint32_t i) {
-  return *(const SH_GLOBAL T*)((const SH_GLOBAL char*)base + i * (uint32_t)sizeof(T));
-}
 
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -194,8 +171,10 @@ struct Pass {
   __device__ __forceinline__ void store(uint32_t (&v)[E]) const {
     constexpr int R = 1 << W;
     if (A.scale) {
+      // x 2^s mod 2^31 - 1 is a rotation of the 31-bit value: canonical in, canonical out
+      const uint32_t sl = A.scale, sr = 31 - A.scale;
 #pragma unroll
-      for (int e = 0; e < E; e++) v[e] = mul(v[e], A.scale);
+      for (int e = 0; e < E; e++) v[e] = ((v[e] << sl) & P) | (v[e] >> sr);
     }
 #pragma unroll
     for (int gi = 0; gi < E / R; gi++) {
@@ -553,8 +532,8 @@ void interpolate_same_size(uint32_t* const* d_cols, uint32_t n_cols, uint32_t lo
   auto ps = plan(log_n, kc);
   for (size_t i = 0; i < ps.size(); i++) {
     A.src = i == 0 ? d_src : (const uint32_t* const*)d_cols;
-    // scale by 2^-log_n in the last pass: 2^-n = 2^(31 - n) since 2^31 = 1
-    A.scale = (i + 1 == ps.size()) ? (1u << (31 - log_n)) % P : 0u;
+    // scale by 2^-log_n in the last pass: 2^-n = 2^(31 - n) since 2^31 = 1 (log_n >= 3 here)
+    A.scale = (i + 1 == ps.size()) ? 31 - log_n : 0u;
     launch_pass<true>(ps[i], A);
   }
 }
@@ -631,7 +610,7 @@ void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint
   Ai.t = t;
   Ai.hb = 0;
   Ai.multi = 0;
-  Ai.scale = (1u << (31 - n)) % P;
+  Ai.scale = 31 - n;
   Ai.n_groups = n_cols;
   Ai.lob_count_log = 12 - t;
   PassArgs Af = Ai;
