@@ -257,13 +257,17 @@ int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_index, uint3
  * plan keeps device copies of every pointer table; the buffers must outlive it.
  * run(use_graph = 1) captures the whole sequence as a hipGraph on first use and replays
  * it; autotune = 1 times the candidate FFT pass schedules of every size class once and
- * keeps the fastest (process-wide, per log size, blowup and column-count bucket). */
+ * keeps the fastest (process-wide, per log size, blowup and column-count bucket), then
+ * times graph replay against stream launches of the plan: where the streams are faster
+ * run(1) launches from the streams too (uses_graph reports the choice).  No counterpart
+ * in the reference (its commit is one synchronous call per operation). */
 typedef struct StwoCommitPlan StwoCommitPlan;
 int stwo_hip_commit_plan_create(StwoCommitPlan **plan, uint32_t n_cols, const uint32_t *const *evals,
                                 uint32_t *const *coeffs, uint32_t *const *ldes, const uint32_t *log_sizes,
                                 uint32_t log_blowup, const uint32_t *twiddles, const uint32_t *itwiddles,
                                 uint64_t tree_len, StwoBlake2sHash *const *layers, uint32_t max_log, int autotune);
 int stwo_hip_commit_plan_run(StwoCommitPlan *plan, int use_graph);
+int stwo_hip_commit_plan_uses_graph(const StwoCommitPlan *plan, int *uses_graph);
 int stwo_hip_commit_plan_destroy(StwoCommitPlan *plan);
 /* The FFT pass schedule in use for an LDE size class (autotuned or default): layers per
  * contiguous pass (12 or 13) and whether the strided passes are fused. */

@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "field.hpp"
+#include "gmem.hpp"
 #include "merkle_plan.hpp"
 #include "runtime.hpp"
 
@@ -182,10 +183,24 @@ __device__ __forceinline__ void quad_compress(uint32_t& h_lo, uint32_t& h_hi, co
 }
 #undef QG
 // Narrow layer, one quad per node (4 x size lanes): children compress, then the 16-column
-// chunks, each staged through the quad's 16 LDS words.
+// chunks, each staged through the quad's 16 LDS words.  With STAGED the block first copies
+// the column-pointer table to LDS: a lane's pointer then comes from an LDS read instead of
+// a global load, so waiting for it does not drain the column loads already in flight
+// (vector-memory loads complete in order).
+constexpr uint32_t QUAD_STAGED_MAX_COLS = 4096;  // 32 KiB of pointers
+template <bool STAGED>
 __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t n_cols, const uint32_t* const* cols,
                                                          const Hash* prev, Hash* out) {
   __shared__ uint32_t msg[64][16];
+  extern __shared__ const uint32_t* sp[];  // STAGED: the n_cols column pointers
+  if constexpr (STAGED) {
+    for (uint32_t j = threadIdx.x; j < n_cols; j += blockDim.x) sp[j] = cols[j];
+    __syncthreads();
+  }
+  auto col_ptr = [&](uint32_t col) -> const uint32_t* {
+    if constexpr (STAGED) return sp[col];
+    else return cols[col];
+  };
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t i = t >> 2, q = t & 3, lq = threadIdx.x >> 2;
   if (i >= size) return;  // whole quads leave together
@@ -200,23 +215,35 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
     __builtin_amdgcn_wave_barrier();
     quad_compress(h_lo, h_hi, mb, q);
   }
-  // column chunks: the next chunk's loads are in flight while the current one is hashed
-  uint32_t nx[4];
+  // column chunks: the loads of the next PF chunks are in flight while one is hashed (a
+  // narrow layer has few waves, so a chunk's HBM latency is hidden only by distance)
+  constexpr int PF = 4;
+  uint32_t nx[PF][4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) nx[k] = 4 * q + k < n_cols ? cols[4 * q + k][i] : 0u;
-  for (uint32_t c = 0; c < n_cols; c += 16) {
-    uint32_t cur[4];
+  for (int d = 0; d < PF; d++)
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      cur[k] = nx[k];
-      const uint32_t col = c + 16 + 4 * q + k;
-      nx[k] = col < n_cols ? cols[col][i] : 0u;
+      const uint32_t col = 16 * d + 4 * q + k;
+      nx[d][k] = col < n_cols ? at(as_global(col_ptr(col)), i) : 0u;
     }
-    __builtin_amdgcn_wave_barrier();  // the quad's reads of the previous message are done
+  for (uint32_t c0 = 0; c0 < n_cols; c0 += 16 * PF) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) mb[4 * q + k] = cur[k];
-    __builtin_amdgcn_wave_barrier();
-    quad_compress(h_lo, h_hi, mb, q);
+    for (int d = 0; d < PF; d++) {
+      const uint32_t c = c0 + 16 * d;
+      if (c >= n_cols) break;  // uniform
+      uint32_t cur[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        cur[k] = nx[d][k];
+        const uint32_t col = c + 16 * PF + 4 * q + k;
+        nx[d][k] = col < n_cols ? at(as_global(col_ptr(col)), i) : 0u;
+      }
+      __builtin_amdgcn_wave_barrier();  // the quad's reads of the previous message are done
+#pragma unroll
+      for (int k = 0; k < 4; k++) mb[4 * q + k] = cur[k];
+      __builtin_amdgcn_wave_barrier();
+      quad_compress(h_lo, h_hi, mb, q);
+    }
   }
   out[i].s[q] = h_lo;
   out[i].s[4 + q] = h_hi;
@@ -379,10 +406,16 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
 }
 
 void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
+  size_t next = 0;
+  enqueue_from(&next, 0, layer_ready);
+}
+
+void MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready) const {
   hipStream_t st = ctx().cur;
-  for (const Step& s : steps) {
+  for (; *next < steps.size() && lowest_layer(steps[*next]) >= min_layer; ++*next) {
+    const Step& s = steps[*next];
     if (layer_ready) {  // the columns injected into the layers of this step must be written
-      const uint32_t lo = s.kind == Step::LAYER ? s.top : s.top + 1 - s.depth;
+      const uint32_t lo = lowest_layer(s);
       for (uint32_t k = lo; k <= s.top; k++)
         if ((*layer_ready)[k]) hip_check(hipStreamWaitEvent(st, (*layer_ready)[k], 0), "hipStreamWaitEvent");
     }
@@ -391,8 +424,13 @@ void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
       const Hash* prev = k == max_log ? nullptr : outs[k + 1];
       timed(k == max_log ? "merkle_leaf" : "merkle_layer", [&] {
         if (k <= QUAD_LAYER_MAX)
-          quad_layer_kernel<<<blocks_for(4ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev,
-                                                                          outs[k]);
+        {
+          const uint32_t nc = lcs[k].n_cols, blocks = blocks_for(4ull << k, 256);
+          if (nc >= 32 && nc <= QUAD_STAGED_MAX_COLS)
+            quad_layer_kernel<true><<<blocks, 256, nc * sizeof(void*), st>>>(1u << k, nc, lcs[k].cols, prev, outs[k]);
+          else
+            quad_layer_kernel<false><<<blocks, 256, 0, st>>>(1u << k, nc, lcs[k].cols, prev, outs[k]);
+        }
         else
           layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev,
                                                                      outs[k]);

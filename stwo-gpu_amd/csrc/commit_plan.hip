@@ -14,7 +14,8 @@
 // Autotuner: for each size class of log >= 14 the candidate pass schedules
 // (contiguous passes of 12 or 13 layers, strided passes fused or not) are timed on
 // the plan's own buffers and the fastest is kept, process-wide, for that
-// (log size, blowup, column-count bucket).
+// (log size, blowup, column-count bucket); then graph replay and stream launches of the
+// whole plan are timed and run(use_graph) uses the graph only where it is faster.
 #include <map>
 #include <memory>
 #include <tuple>
@@ -55,6 +56,7 @@ struct CommitPlan {
   std::unique_ptr<merkle::MerklePlan> merkle;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool graph_pays = true;  // autotuned: replaying the graph beats launching from streams
   std::vector<hipEvent_t> class_done;  // one per class
 
   void enqueue_class(const Class& c) const {
@@ -66,7 +68,9 @@ struct CommitPlan {
   // Classes run on the auxiliary streams, largest first, round robin; each records an
   // event when its LDE columns are written, and every Merkle launch waits only for the
   // classes whose columns it hashes, so the top of the tree is built while smaller
-  // classes are still being transformed.  Every auxiliary stream's last event is waited
+  // classes are still being transformed.  Merkle launches are submitted as soon as the
+  // classes they read are submitted (not after every class), so the device sees them
+  // early.  Every auxiliary stream's last event is waited
   // for by some Merkle launch, which joins the streams back (as graph capture needs).
   void enqueue() const {
     Context& x = ctx();
@@ -79,14 +83,19 @@ struct CommitPlan {
       hip_check(hipEventRecord(x.fork, x.stream), "hipEventRecord");
       for (int i = 0; i < n_aux; i++) hip_check(hipStreamWaitEvent(x.aux[i], x.fork, 0), "hipStreamWaitEvent");
       std::vector<hipEvent_t> ready(merkle->max_log + 1, nullptr);
+      size_t next_step = 0;
       for (size_t i = 0; i < classes.size(); i++) {
         x.cur = x.aux[i % n_aux];
         enqueue_class(classes[i]);
         hip_check(hipEventRecord(class_done[i], x.cur), "hipEventRecord");
         ready[classes[i].log + log_blowup] = class_done[i];
+        // the Merkle steps above the next (smaller) class read only classes enqueued so far
+        x.cur = x.stream;
+        const uint32_t below = i + 1 < classes.size() ? classes[i + 1].log + log_blowup + 1 : 0;
+        merkle->enqueue_from(&next_step, below, &ready);
       }
       x.cur = x.stream;
-      merkle->enqueue(&ready);
+      merkle->enqueue_from(&next_step, 0, &ready);
     } else {
       for (const Class& c : classes) enqueue_class(c);
       merkle->enqueue();
@@ -95,7 +104,7 @@ struct CommitPlan {
 
   void run(bool use_graph) {
     Context& x = ctx();
-    if (!use_graph) {
+    if (!use_graph || !graph_pays) {
       enqueue();
       return;
     }
@@ -151,6 +160,26 @@ struct CommitPlan {
       c.var = best_v;
       fft::set_lde_variant(c.log, log_blowup, c.n, best_v);
     }
+    // graph replay against stream launches of the whole plan (a plan of many size classes
+    // can keep more of them in flight from the streams than its replayed graph does)
+    auto time_runs = [&](bool g) {
+      run(g);  // warm-up (captures the graph on first use)
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 3; rep++) {
+        hip_check(hipEventRecord(x.timer_start, x.stream), "hipEventRecord");
+        run(g);
+        hip_check(hipEventRecord(x.timer_stop, x.stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(x.timer_stop), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, x.timer_start, x.timer_stop), "hipEventElapsedTime");
+        ms_min = std::min(ms_min, ms);
+      }
+      return ms_min;
+    };
+    graph_pays = true;
+    const float ms_graph = time_runs(true);
+    const float ms_stream = time_runs(false);
+    graph_pays = ms_graph <= ms_stream;
   }
 
   ~CommitPlan() {
@@ -227,6 +256,13 @@ extern "C" int stwo_hip_lde_schedule(uint32_t log_size, uint32_t log_blowup, uin
     const fft::LdeVariant v = fft::lde_variant_for(log_size, log_blowup, n_cols);
     if (contiguous_layers) *contiguous_layers = v.kc;
     if (fused) *fused = v.fused ? 1 : 0;
+  });
+}
+
+extern "C" int stwo_hip_commit_plan_uses_graph(const StwoCommitPlan* plan, int* uses_graph) {
+  return guarded([&] {
+    require(plan != nullptr && uses_graph != nullptr, "commit plan: null argument");
+    *uses_graph = plan->plan.graph_pays ? 1 : 0;
   });
 }
 

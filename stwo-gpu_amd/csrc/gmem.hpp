@@ -1,5 +1,5 @@
-// Global-memory access helpers for the FFT kernels, which read columns through device
-// pointer tables.  A pointer loaded from a table is generic; casting it to the global address
+// Global-memory access helpers for kernels that read columns through device pointer
+// tables.  A pointer loaded from a table is generic; casting it to the global address
 // space lets the compiler emit global_* instead of flat_* accesses, and forming the byte
 // offset in 32 bits (the caller guarantees offsets below 2^32: M31 columns hold at most
 // 2^30 values) gives one load with a scalar base and a 32-bit vector offset instead of

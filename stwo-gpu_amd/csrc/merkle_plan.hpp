@@ -53,6 +53,13 @@ struct MerklePlan {
   // each step waits only for the layers it reads, so hashing starts while other
   // columns are still being produced.
   void enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr) const;
+  // Enqueue steps from *next on while every layer they read is >= min_layer (the caller
+  // has enqueued the producers of those layers); advances *next.  Interleaving these
+  // calls with the producers' launches hands the hashing to the device as early as the
+  // host submits its inputs.
+  void enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready) const;
+  // lowest layer a step reads injected columns of
+  static uint32_t lowest_layer(const Step& s) { return s.kind == Step::LAYER ? s.top : s.top + 1 - s.depth; }
 
  private:
   void build(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers, char* dev,

@@ -121,6 +121,7 @@ def _declare(L):
         "stwo_hip_commit_plan_create": (C.c_int, [C.POINTER(vp), u32, pp, pp, pp, u32p, u32, vp, vp, u64, pp, u32,
                                                   C.c_int]),
         "stwo_hip_commit_plan_run": (C.c_int, [vp, C.c_int]),
+        "stwo_hip_commit_plan_uses_graph": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "stwo_hip_commit_plan_destroy": (C.c_int, [vp]),
         "stwo_hip_lde_schedule": (C.c_int, [u32, u32, u64, u32p, C.POINTER(C.c_int)]),
         "stwo_hip_grind": (C.c_int, [u32p, u32, C.POINTER(u64)]),
@@ -684,6 +685,12 @@ class CommitPlan:
 
     def run(self, use_graph=True):
         _c(lib().stwo_hip_commit_plan_run(self.handle, 1 if use_graph else 0))
+
+    def uses_graph(self):
+        """Whether run(True) replays a hipGraph (False: the autotuner found stream launches faster)."""
+        v = C.c_int(0)
+        _c(lib().stwo_hip_commit_plan_uses_graph(self.handle, C.byref(v)))
+        return bool(v.value)
 
     def root(self):
         out = np.zeros(8, np.uint32)
