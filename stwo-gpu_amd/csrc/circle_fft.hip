@@ -44,6 +44,7 @@ struct PassArgs {
   uint32_t multi;    // 1: the hb bits select a column (whole small columns per tile)
   uint32_t scale;    // 0: none; else multiply outputs by 2^scale (1 <= scale <= 30)
   uint32_t n_groups;     // column groups (grid x = n_groups * tiles_per_group)
+  uint32_t band;         // consecutive blocks walk `band` columns at one tile, then the next tile
   uint32_t lob_count_log;  // log2 of the number of low blocks per column (a - t)
 };
 
@@ -85,8 +86,19 @@ struct Pass {
   SH_GLOBAL uint32_t* dst;
 
   __device__ __forceinline__ Pass(const PassArgs& a, uint32_t* lds) : A(a), s(lds) {
-    col_group = blockIdx.x % A.n_groups;
-    tile = blockIdx.x / A.n_groups;
+    if constexpr (KIND == MULTI) {
+      col_group = blockIdx.x % A.n_groups;
+      tile = blockIdx.x / A.n_groups;
+    } else {
+      // bands of `band` columns: the blocks in flight share each tile's twiddles across the
+      // band and touch few columns (pages) at a time
+      const uint32_t tiles = gridDim.x / A.n_groups;
+      const uint32_t per_band = A.band * tiles;
+      const uint32_t bi = blockIdx.x / per_band, r = blockIdx.x - bi * per_band;
+      const uint32_t w = min(A.band, A.n_groups - bi * A.band);
+      col_group = bi * A.band + r % w;
+      tile = r / w;
+    }
     if constexpr (KIND == MULTI) {
       block_base = 0;
       hi_tile = 0;
@@ -414,6 +426,9 @@ __global__ void fft_tiny_kernel(uint32_t* const* dst, const uint32_t* const* src
 // log size <= 12 are grouped several to a tile.
 // ---------------------------------------------------------------------------
 constexpr int BS = 14;  // strided tile bits (64 KiB of LDS)
+// columns per block band (8 measured ~1 % faster than one band over all 256 columns of
+// log 20)
+constexpr uint32_t FFT_BAND = 8;
 constexpr uint32_t KS_MAX = BS - 5;
 
 struct PassPlan {
@@ -493,6 +508,7 @@ static void launch_pass(const PassPlan& p, const PassArgs& base) {
     blocks = A.n_groups;
   } else {
     A.n_groups = A.n_cols;
+    A.band = std::min(A.n_cols, FFT_BAND);
     A.lob_count_log = p.a - p.t;
     uint32_t hi_log = A.log_n - p.a - p.k - p.hb;
     blocks = A.n_cols << (A.lob_count_log + hi_log);
@@ -612,6 +628,7 @@ void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint
   Ai.multi = 0;
   Ai.scale = 31 - n;
   Ai.n_groups = n_cols;
+  Ai.band = std::min(n_cols, FFT_BAND);
   Ai.lob_count_log = 12 - t;
   PassArgs Af = Ai;
   Af.src = (const uint32_t* const*)d_lde;
