@@ -311,13 +311,13 @@ __global__ __launch_bounds__(NT) void fft_pass_kernel(PassArgs A) {
 // and forward strided passes (which read the coefficients back).
 constexpr int BS_LDE = 14;
 // The fused middle pass saves one HBM read of the coefficients, but it holds the
-// coefficients and the forward tile in registers (4 waves/SIMD instead of 8); on gfx950
-// the separate strided passes measured faster (1.63 ms vs 1.73 ms for 256 columns of log
-// 20), so it is a variant for the autotuner rather than the default.
+// coefficients and the forward tile in registers.
 constexpr int NT_S = 1024;  // threads per 2^14-element tile: 16 values per thread
 // Tile bits of the fused pass for K layers: 2^13-element tiles (8 values per thread, 32-word
 // runs) while K <= 8, so the kept coefficients and the forward copy fit in 64 VGPRs and two
-// blocks share a CU; 2^14 for K = 9.
+// blocks share a CU (256 columns of log 20: 1.39 ms against 1.63 ms for the separate
+// strided passes, the default for log 14..20); 2^14 for K = 9 (98 VGPRs, one block per
+// CU: a variant for the autotuner only).
 template <int K>
 constexpr int lde_tile_bits() {
   return K <= 8 ? 13 : BS_LDE;

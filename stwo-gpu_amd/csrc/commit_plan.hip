@@ -35,7 +35,12 @@ uint32_t bucket(uint64_t n_cols) { return log2_exact(n_cols ? n_cols : 1); }
 
 LdeVariant lde_variant_for(uint32_t n, uint32_t log_blowup, uint64_t n_cols) {
   auto it = g_variants.find({n, log_blowup, bucket(n_cols)});
-  return it == g_variants.end() ? LdeVariant{} : it->second;
+  if (it != g_variants.end()) return it->second;
+  // untuned default: the fused middle pass where it runs on 2^13-element tiles (at most 8
+  // strided layers, log sizes 14..20), measured faster than separate strided passes there
+  LdeVariant v;
+  v.fused = n >= 14 && n <= 20;
+  return v;
 }
 void set_lde_variant(uint32_t n, uint32_t log_blowup, uint64_t n_cols, LdeVariant v) {
   g_variants[{n, log_blowup, bucket(n_cols)}] = v;
