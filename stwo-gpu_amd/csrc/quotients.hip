@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "field.hpp"
+#include "gmem.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         const Sample S = samples[B.first + j + u];
-        uint64_t v = cols[S.col][row];
+        uint64_t v = at(as_global(cols[S.col]), row);
         s0 += S.alpha.x.a * v;
         s1 += S.alpha.x.b * v;
         s2 += S.alpha.y.a * v;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
     }
     if (j < B.count) {
       const Sample S = samples[B.first + j];
-      uint64_t v = cols[S.col][row];
+      uint64_t v = at(as_global(cols[S.col]), row);
       s0 += S.alpha.x.a * v;
       s1 += S.alpha.x.b * v;
       s2 += S.alpha.y.a * v;
@@ -121,15 +122,15 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
   uint64_t s[NB][4];
 #pragma unroll
   for (int b = 0; b < NB; b++) s[b][0] = s[b][1] = s[b][2] = s[b][3] = 0;
-  // n_slots is padded to a multiple of QG with zero-alpha slots of column 0
+  // n_slots is a multiple of QG (zero-alpha slots of column 0 pad it), and QG more such
+  // slots follow, so the next group's loads are issued unconditionally: a conditional
+  // prefetch makes the compiler wait for every load in flight before the arithmetic
   uint32_t v[QG], vn[QG];
 #pragma unroll
-  for (int i = 0; i < QG; i++) v[i] = slots[i].col[row];
+  for (int i = 0; i < QG; i++) v[i] = at(as_global(slots[i].col), row);
   for (uint32_t h = 0; h < n_slots; h += QG) {
-    if (h + QG < n_slots) {
 #pragma unroll
-      for (int i = 0; i < QG; i++) vn[i] = slots[h + QG + i].col[row];
-    }
+    for (int i = 0; i < QG; i++) vn[i] = at(as_global(slots[h + QG + i].col), row);
 #pragma unroll
     for (int i = 0; i < QG; i++) {
       const ColSlot<NB>& S = slots[h + i];
@@ -202,9 +203,12 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* con
 }  // namespace quot
 
 static uint32_t padded_slots(uint32_t n_cols) { return (n_cols + quot::QG - 1) / quot::QG * quot::QG; }
+// slots: the padded loop range plus QG trailing pad slots
+static size_t slot_bytes(uint32_t n_slots) {
+  return (sizeof(quot::ColSlot<1>) * (n_slots + quot::QG) + 255) & ~(size_t)255;
+}
 size_t quotient_single_point_scratch_bytes(uint32_t n_cols) {
-  return ((sizeof(quot::ColSlot<1>) * padded_slots(std::max<uint32_t>(n_cols, 1)) + 255) & ~(size_t)255) +
-         sizeof(quot::Batch);
+  return slot_bytes(padded_slots(std::max<uint32_t>(n_cols, 1))) + sizeof(quot::Batch);
 }
 
 // accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
@@ -216,11 +220,10 @@ void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t 
   require(log_size >= 1 && log_size <= 31 && n_cols >= 1, "accumulate_quotients: bad sizes");
   const uint32_t n_slots = padded_slots(n_cols);
   quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
-  quot::Batch* bs =
-      (quot::Batch*)((char*)d_scratch + ((sizeof(quot::ColSlot<1>) * n_slots + 255) & ~(size_t)255));
+  quot::Batch* bs = (quot::Batch*)((char*)d_scratch + slot_bytes(n_slots));
   timed("quotient_constants", [&] {
-    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss, n_slots,
-                                                                bs);
+    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss,
+                                                                n_slots + quot::QG, bs);
   });
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<1><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
@@ -253,12 +256,13 @@ void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t*
   pad.col = slots.empty() ? h_cols[0] : slots[0].col;
   for (int k = 0; k < NB; k++) pad.alpha[k] = qzero();
   while (slots.empty() || slots.size() % quot::QG) slots.push_back(pad);
+  const uint32_t n_loop = (uint32_t)slots.size();
+  for (int i = 0; i < quot::QG; i++) slots.push_back(pad);  // read by the last group's prefetch
   DevBuf dsl = upload(slots.data(), slots.size() * sizeof(quot::ColSlot<NB>));
   DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<NB><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        hi, hs, log_size, dsl.as<quot::ColSlot<NB>>(), (uint32_t)slots.size(), dbs.as<quot::Batch>(), out[0], out[1],
-        out[2], out[3]);
+        hi, hs, log_size, dsl.as<quot::ColSlot<NB>>(), n_loop, dbs.as<quot::Batch>(), out[0], out[1], out[2], out[3]);
   });
 }
 void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "field.hpp"
+#include "gmem.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -29,15 +30,17 @@ __global__ __launch_bounds__(256) void quotients_kernel(uint32_t n_rows, uint32_
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  uint32_t sa = sqr(trace[0][r]), sb = sqr(trace[1][r]);
-  // constraint i reads column i + 2; columns are loaded one group ahead of the arithmetic
-  const uint32_t n_cons = n_cols - 2;
+  uint32_t sa = sqr(at(as_global(trace[0]), r)), sb = sqr(at(as_global(trace[1]), r));
+  // constraint i reads column i + 2; columns are loaded one group ahead of the arithmetic,
+  // unconditionally (indices past the last column re-read it and are never used), so the
+  // arithmetic waits only for the older group
+  const uint32_t n_cons = n_cols - 2, last = n_cols - 1;
   uint32_t cv[WG], cn[WG];
 #pragma unroll
-  for (int g = 0; g < WG; g++) cv[g] = (uint32_t)g < n_cons ? trace[2 + g][r] : 0u;
+  for (int g = 0; g < WG; g++) cv[g] = at(as_global(trace[min(2u + g, last)]), r);
   for (uint32_t i = 0; i < n_cons; i += WG) {
 #pragma unroll
-    for (int g = 0; g < WG; g++) cn[g] = i + WG + g < n_cons ? trace[2 + i + WG + g][r] : 0u;
+    for (int g = 0; g < WG; g++) cn[g] = at(as_global(trace[min(2u + i + WG + g, last)]), r);
 #pragma unroll
     for (int g = 0; g < WG; g++) {
       if (i + g >= n_cons) break;
@@ -77,11 +80,11 @@ __global__ __launch_bounds__(256) void trace_kernel(uint32_t* const* __restrict_
   if (r >= n) return;
   const uint64_t seed = *seed_dev;
   uint32_t a = uniform_m31(seed, r), b = uniform_m31(seed, (uint64_t)n + r);
-  cols[0][r] = a;
-  cols[1][r] = b;
+  at(as_global(cols[0]), r) = a;
+  at(as_global(cols[1]), r) = b;
   for (uint32_t i = 2; i < n_cols; i++) {
     const uint32_t c = add(sqr(a), sqr(b));
-    cols[i][r] = c;
+    at(as_global(cols[i]), r) = c;
     a = b;
     b = c;
   }
