@@ -126,19 +126,62 @@ struct Pass {
   }
   __device__ __forceinline__ uint32_t col_of(uint32_t p) const { return (col_group << HB) + (p >> K); }
 
-  // twiddle of layer bit b at the butterfly block containing position p
+  // twiddle of layer bit b at the butterfly block containing position p: the word loaded
+  // from its level (tw_index) and the sign applied after the load (tw_fix)
   template <int BIT>
-  __device__ __forceinline__ uint32_t twiddle(const SH_GLOBAL uint32_t* lvl, uint32_t p) const {
+  __device__ __forceinline__ uint32_t tw_h(uint32_t p) const {
     const uint32_t m = (p >> T) & ((1u << K) - 1);
-    const uint32_t h = (hi_tile << (K - BIT - 1)) | (m >> (BIT + 1));
+    return (hi_tile << (K - BIT - 1)) | (m >> (BIT + 1));
+  }
+  template <int BIT>
+  __device__ __forceinline__ uint32_t tw_index(uint32_t p) const {
+    const uint32_t h = tw_h<BIT>(p);
     if constexpr (KIND != STRIDED && BIT == 0) {
       // circle layer: [y, -y, -x, x] over pairs (x, y) of the first line level
-      const uint32_t k2 = (h >> 2) << 1, sel = h & 3;
-      const uint32_t v = at(lvl, k2 + (sel < 2 ? 1 : 0));
+      return ((h >> 2) << 1) + ((h & 3) < 2 ? 1 : 0);
+    } else {
+      return h;
+    }
+  }
+  template <int BIT>
+  __device__ __forceinline__ uint32_t tw_fix(uint32_t v, uint32_t p) const {
+    if constexpr (KIND != STRIDED && BIT == 0) {
+      const uint32_t sel = tw_h<BIT>(p) & 3;
       return (sel == 1 || sel == 2) ? neg(v) : v;
     } else {
-      return at(lvl, h);
+      return v;
     }
+  }
+  template <int BIT>
+  __device__ __forceinline__ uint32_t twiddle(const SH_GLOBAL uint32_t* lvl, uint32_t p) const {
+    return tw_fix<BIT>(at(lvl, tw_index<BIT>(p)), p);
+  }
+
+  // Twiddles of one radix-2^W stage held in registers (at most 7 per 8 values): slot of
+  // layer bit BL, value group gi, butterfly block rh.
+  static constexpr int NTW = E - E / 8;
+  template <int W, int BL>
+  static constexpr int tw_slot(int gi, int rh) {
+    return (E >> W) * ((1 << W) - ((1 << W) >> BL)) + gi * ((1 << W) >> (BL + 1)) + rh;
+  }
+  template <int LO, int W>
+  __device__ __forceinline__ void load_tw(uint32_t (&tw)[NTW]) const {
+    constexpr int R = 1 << W;
+    static_for<0, W>([&](auto si) {
+      constexpr int BL = si.value;
+      constexpr int BIT = LO + BL;
+      if constexpr (MASKED) {
+        if (A.a + BIT >= A.act_log) return;  // layer skipped by stage()
+      }
+      const SH_GLOBAL uint32_t* lvl = level(BIT);
+#pragma unroll
+      for (int gi = 0; gi < E / R; gi++) {
+        const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+#pragma unroll
+        for (int rh = 0; rh < (R >> (BL + 1)); rh++)
+          tw[tw_slot<W, BL>(gi, rh)] = at(lvl, tw_index<BIT>(pb | ((uint32_t)(rh << (BL + 1)) << (T + LO))));
+      }
+    });
   }
   // line level used by layer bit b (global layer a + b >= 1); level 0 for the circle layer
   __device__ __forceinline__ const SH_GLOBAL uint32_t* level(uint32_t b) const {
@@ -234,9 +277,10 @@ struct Pass {
     }
   }
 
-  // radix-2^W butterflies on layer bits [LO, LO + W); forward passes skip layers >= act_log
+  // radix-2^W butterflies on layer bits [LO, LO + W); forward passes skip layers >= act_log.
+  // pre: the stage's twiddles loaded by load_tw (else each is loaded where it is used).
   template <int LO, int W>
-  __device__ __forceinline__ void stage(uint32_t (&v)[E]) const {
+  __device__ __forceinline__ void stage(uint32_t (&v)[E], const uint32_t* pre = nullptr) const {
     constexpr int R = 1 << W;
     static_for<0, W>([&](auto si) {
       constexpr int BL = INV ? si.value : (W - 1 - si.value);
@@ -250,7 +294,8 @@ struct Pass {
         const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
 #pragma unroll
         for (int rh = 0; rh < (R >> (BL + 1)); rh++) {
-          const uint32_t tw = twiddle<BIT>(lvl, pb | ((uint32_t)(rh << (BL + 1)) << (T + LO)));
+          const uint32_t p = pb | ((uint32_t)(rh << (BL + 1)) << (T + LO));
+          const uint32_t tw = pre ? tw_fix<BIT>(pre[tw_slot<W, BL>(gi, rh)], p) : twiddle<BIT>(lvl, p);
 #pragma unroll
           for (int rl = 0; rl < (1 << BL); rl++) {
             const int e0 = gi * R + (rh << (BL + 1)) + rl;
@@ -270,19 +315,26 @@ struct Pass {
     });
   }
 
+  // The twiddles of the next stage are loaded before the LDS exchange, so their latency
+  // overlaps the barriers instead of following them.
   __device__ __forceinline__ void run() {
-    uint32_t v[E];
+    uint32_t v[E], tw[NTW];
     static_for<0, NCH>([&](auto it) {
       constexpr int c = INV ? it.value : NCH - 1 - it.value;
       constexpr int LO = 3 * c;
       constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
       if constexpr (it.value == 0) {
+        load_tw<LO, W>(tw);
         load<LO, W>(v);
       } else {
         lds_read<LO, W>(v);
       }
-      stage<LO, W>(v);
+      stage<LO, W>(v, tw);
       if constexpr (it.value + 1 < NCH) {
+        constexpr int cn = INV ? it.value + 1 : NCH - 2 - it.value;
+        constexpr int LON = 3 * cn;
+        constexpr int WN = (K - LON) < 3 ? (K - LON) : 3;
+        load_tw<LON, WN>(tw);
         if constexpr (it.value > 0) __syncthreads();  // previous round's reads are done
         lds_write<LO, W>(v);
         __syncthreads();
