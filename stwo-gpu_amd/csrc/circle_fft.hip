@@ -383,19 +383,24 @@ void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
   using PF = Pass<false, BL, NT_S, K, STRIDED, false>;
   constexpr int E = PI::E, NCH = PI::NCH;
   constexpr int LO_TOP = 3 * (NCH - 1), W_TOP = K - LO_TOP;
+  // each stage's twiddles are loaded before the exchange that precedes it (as in Pass::run)
   PI pi(Ai, s);
-  uint32_t v[E];
+  uint32_t v[E], tw[PI::NTW];
   static_for<0, NCH>([&](auto it) {
     constexpr int c = it.value;
     constexpr int LO = 3 * c;
     constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
     if constexpr (c == 0) {
+      pi.template load_tw<LO, W>(tw);
       pi.template load<LO, W>(v);
     } else {
       pi.template lds_read<LO, W>(v);
     }
-    pi.template stage<LO, W>(v);
+    pi.template stage<LO, W>(v, tw);
     if constexpr (c + 1 < NCH) {
+      constexpr int LON = 3 * (c + 1);
+      constexpr int WN = (K - LON) < 3 ? (K - LON) : 3;
+      pi.template load_tw<LON, WN>(tw);
       if constexpr (c > 0) __syncthreads();
       pi.template lds_write<LO, W>(v);
       __syncthreads();
@@ -412,9 +417,13 @@ void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
       constexpr int c = NCH - 1 - it.value;
       constexpr int LO = 3 * c;
       constexpr int W = (K - LO) < 3 ? (K - LO) : 3;
+      if constexpr (it.value == 0) pf.template load_tw<LO, W>(tw);
       if constexpr (it.value > 0) pf.template lds_read<LO, W>(w);
-      pf.template stage<LO, W>(w);
+      pf.template stage<LO, W>(w, tw);
       if constexpr (c > 0) {
+        constexpr int LON = 3 * (c - 1);
+        constexpr int WN = (K - LON) < 3 ? (K - LON) : 3;
+        pf.template load_tw<LON, WN>(tw);
         __syncthreads();  // earlier readers of s are done
         pf.template lds_write<LO, W>(w);
         __syncthreads();
