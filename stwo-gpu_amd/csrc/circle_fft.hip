@@ -143,13 +143,16 @@ struct Pass {
       return h;
     }
   }
+  // the butterflies multiply by twice the twiddle (mul_dbl); a negated circle twiddle
+  // -t is represented by 2P - 2t (that is 2(P - t), with t = 0 giving the factor 2P)
   template <int BIT>
   __device__ __forceinline__ uint32_t tw_fix(uint32_t v, uint32_t p) const {
+    const uint32_t v2 = v << 1;
     if constexpr (KIND != STRIDED && BIT == 0) {
       const uint32_t sel = tw_h<BIT>(p) & 3;
-      return (sel == 1 || sel == 2) ? neg(v) : v;
+      return (sel == 1 || sel == 2) ? 2 * P - v2 : v2;
     } else {
-      return v;
+      return v2;
     }
   }
   template <int BIT>
@@ -303,9 +306,9 @@ struct Pass {
             const uint32_t x0 = v[e0], x1 = v[e1];
             if constexpr (INV) {
               v[e0] = add(x0, x1);
-              v[e1] = mul(sub(x0, x1), tw);
+              v[e1] = mul_dbl(sub(x0, x1), tw);
             } else {
-              const uint32_t t = mul(x1, tw);
+              const uint32_t t = mul_dbl(x1, tw);
               v[e0] = add(x0, t);
               v[e1] = sub(x0, t);
             }
