@@ -42,6 +42,28 @@ def algorithmic_bytes(C, n):
     }
 
 
+# VALU roofline: one 32-bit VALU instruction per lane per cycle on 64 lanes of each of 256
+# CUs at 2.4 GHz.  Algorithmic instruction counts: a butterfly is 12 lane-instructions
+# (64-bit product by a doubled twiddle, shift, add, and two min-based canonicalisations
+# for the sum and the difference); a Blake2s compression is 960 (10 rounds x 8 G x 12:
+# add3, xor, alignbit per quarter step).
+VALU_PEAK_OPS = 256 * 64 * 2.4e9
+OPS_PER_BUTTERFLY = 12
+OPS_PER_COMPRESSION = 960
+
+
+def algorithmic_valu_ops(C, n):
+    """Per-launch algorithmic VALU lane-instructions of each kernel of a step (blowup 2)."""
+    m = 2 * n
+    kc, ks_inv, ks_fwd = 12, (n.bit_length() - 1) - 12, (m.bit_length() - 1) - 13  # forward skips the top layer
+    return {
+        "fft_inv_contig": OPS_PER_BUTTERFLY * C * (n // 2) * kc,
+        "fft_lde_strided": OPS_PER_BUTTERFLY * C * ((n // 2) * ks_inv + 2 * (n // 2) * ks_fwd),
+        "fft_fwd_contig": OPS_PER_BUTTERFLY * C * (m // 2) * kc,
+        "merkle_leaf": OPS_PER_COMPRESSION * m * ((C + 15) // 16),
+    }
+
+
 def pipeline_bytes(C, n):
     """Whole-step algorithmic bytes: 28 C n + 256 n (interp 8Cn, eval 12Cn, leaves 8Cn + 64n, nodes)."""
     m = 2 * n
@@ -120,6 +142,7 @@ def bench_lde_commit(args):
              "merkle_layer", "merkle_subtree"]
     stats = {k: S.kernel_stats(k) for k in names}
     alg = algorithmic_bytes(C, n)
+    vops = algorithmic_valu_ops(C, n)
     kernels = {}
     for k, (cnt, tot) in stats.items():
         if cnt:
@@ -127,6 +150,8 @@ def bench_lde_commit(args):
             kernels[k] = {"launches_per_step": cnt / reps, "avg_ms": round(avg, 4)}
             if k in alg:
                 kernels[k]["GBps"] = round(alg[k] / (avg * 1e-3) / 1e9, 1)
+            if k in vops:
+                kernels[k]["valu_frac"] = round(vops[k] / (avg * 1e-3) / VALU_PEAK_OPS, 4)
     dom = max((k for k in alg if k in kernels), key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_step"])
     d_avg = kernels[dom]["avg_ms"]
     achieved = alg[dom] / (d_avg * 1e-3) / 1e9
@@ -155,6 +180,12 @@ def bench_lde_commit(args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg[dom], "avg_launch_ms": d_avg},
+        # the same kernel against the VALU roofline (the step is VALU-bound: exact M31 arithmetic)
+        "valu_roofline": {"bound": "valu", "kernel": dom,
+                          "achieved": round(vops[dom] / (d_avg * 1e-3) / 1e12, 2) if dom in vops else None,
+                          "peak": round(VALU_PEAK_OPS / 1e12, 2), "unit": "T int32 lane-ops/s",
+                          "frac": kernels[dom].get("valu_frac"), "ops_per_launch": vops.get(dom),
+                          "ops_per_butterfly": OPS_PER_BUTTERFLY, "ops_per_compression": OPS_PER_COMPRESSION},
         "pipeline": {"algorithmic_bytes_per_step": pipe,
                      "achieved_GBps": round(pipe / (ms_per_step * 1e-3) / 1e9, 1),
                      "frac_of_hbm_peak": round(pipe / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
