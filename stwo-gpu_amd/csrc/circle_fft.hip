@@ -155,10 +155,6 @@ struct Pass {
       return v2;
     }
   }
-  template <int BIT>
-  __device__ __forceinline__ uint32_t twiddle(const SH_GLOBAL uint32_t* lvl, uint32_t p) const {
-    return tw_fix<BIT>(at(lvl, tw_index<BIT>(p)), p);
-  }
 
   // Twiddles of one radix-2^W stage held in registers (at most 7 per 8 values): slot of
   // layer bit BL, value group gi, butterfly block rh.
@@ -281,9 +277,9 @@ struct Pass {
   }
 
   // radix-2^W butterflies on layer bits [LO, LO + W); forward passes skip layers >= act_log.
-  // pre: the stage's twiddles loaded by load_tw (else each is loaded where it is used).
+  // pre: the stage's twiddles, loaded by load_tw<LO, W>.
   template <int LO, int W>
-  __device__ __forceinline__ void stage(uint32_t (&v)[E], const uint32_t* pre = nullptr) const {
+  __device__ __forceinline__ void stage(uint32_t (&v)[E], const uint32_t (&pre)[NTW]) const {
     constexpr int R = 1 << W;
     static_for<0, W>([&](auto si) {
       constexpr int BL = INV ? si.value : (W - 1 - si.value);
@@ -291,14 +287,13 @@ struct Pass {
       if constexpr (MASKED) {
         if (A.a + BIT >= A.act_log) return;  // uniform: layer over zero padding
       }
-      const SH_GLOBAL uint32_t* lvl = level(BIT);
 #pragma unroll
       for (int gi = 0; gi < E / R; gi++) {
         const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
 #pragma unroll
         for (int rh = 0; rh < (R >> (BL + 1)); rh++) {
           const uint32_t p = pb | ((uint32_t)(rh << (BL + 1)) << (T + LO));
-          const uint32_t tw = pre ? tw_fix<BIT>(pre[tw_slot<W, BL>(gi, rh)], p) : twiddle<BIT>(lvl, p);
+          const uint32_t tw = tw_fix<BIT>(pre[tw_slot<W, BL>(gi, rh)], p);
 #pragma unroll
           for (int rl = 0; rl < (1 << BL); rl++) {
             const int e0 = gi * R + (rh << (BL + 1)) + rl;
