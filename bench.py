@@ -565,9 +565,9 @@ def serving_jobs(n_jobs, rate, seed=0x5EED0005):
     return [(float(t[i]), int(cols[i]), int(logs[i]), 0xA000 + i) for i in range(n_jobs)]
 
 
-def _serve(S, jobs, budget, slots, pow_bits, last_log):
+def _serve(S, jobs, budget, slots, pow_bits, last_log, work_limit=0):
     """Submit each job at its arrival time; returns per-job ((submit, start, end), proof words)."""
-    srv = S.ProofServer(budget, slots)
+    srv = S.ProofServer(budget, slots, work_limit)
     t0 = time.perf_counter()
     done, ids = {}, {}
     i = 0
@@ -641,20 +641,21 @@ def bench_serving(args):
                 "p99_ms": round(float(np.percentile(lat, 99) * 1e3), 2), "makespan_s": round(last - first, 4),
                 "host_ms_per_job": round(float(np.mean([done[i][0][3] for i in range(n)])) * 1e3, 3)}
 
-    batched = summarize(_serve(S, jobs, budget, args.serve_slots, pow_bits, last_log), n_jobs)
+    wl = args.serve_work_limit
+    batched = summarize(_serve(S, jobs, budget, args.serve_slots, pow_bits, last_log, wl), n_jobs)
     one_slot = summarize(_serve(S, jobs, budget, 1, pow_bits, last_log), n_jobs)
     plain = summarize(_serve_plain(jobs, pow_bits, last_log), n_jobs)
     sweep = []
     for r in (args.serve_sweep or []):
         js = serving_jobs(n_jobs, r)
         sweep.append({"rate": r,
-                      "batched": summarize(_serve(S, js, budget, args.serve_slots, pow_bits, last_log), n_jobs),
+                      "batched": summarize(_serve(S, js, budget, args.serve_slots, pow_bits, last_log, wl), n_jobs),
                       "device_prover_one_at_a_time": summarize(_serve(S, js, budget, 1, pow_bits, last_log), n_jobs)})
     elems = sum(C << ln for _, C, ln, _ in jobs)
     extra = {"batched": batched, "plain_reference_shaped": plain, "device_prover_one_at_a_time": one_slot,
              "throughput_ratio_vs_plain": round(batched["proofs_per_s"] / plain["proofs_per_s"], 2),
              "throughput_ratio_vs_one_at_a_time": round(batched["proofs_per_s"] / one_slot["proofs_per_s"], 2),
-             "jobs": n_jobs, "arrival_rate_per_s": rate, "slots": args.serve_slots,
+             "jobs": n_jobs, "arrival_rate_per_s": rate, "slots": args.serve_slots, "work_limit": wl,
              "hbm_budget_gib": args.serve_budget_gib, "trace_elements_total": elems,
              "pow_bits": pow_bits, "last_layer_log": last_log, "rate_sweep": sweep}
     return _line("served wide-Fibonacci proofs/s (continuous batching)", batched["proofs_per_s"], "proofs/s",
@@ -681,6 +682,9 @@ def main():
                     help="Poisson arrival rate (jobs/s); the default offers more than one GPU proves, so "
                          "proofs/s is the server's capacity")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-work-limit", type=int, default=1 << 31,
+                    help="trace elements (n_cols x 2^log_n) in flight at most (0: no limit); 2^31, four of "
+                         "the largest jobs, measured best with 8 slots")
     ap.add_argument("--serve-budget-gib", type=int, default=96)
     ap.add_argument("--serve-sweep", type=float, nargs="*", default=[400.0, 1000.0],
                     help="extra arrival rates to run both device modes at (latency under partial load)")

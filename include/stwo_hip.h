@@ -321,6 +321,10 @@ int stwo_hip_wide_fib_prove(const StwoWideFibParams *params, uint64_t seed, uint
  * the host time spent building and enqueuing it. */
 typedef struct StwoProofServer StwoProofServer;
 int stwo_hip_server_create(StwoProofServer **server, uint64_t hbm_budget_bytes, uint32_t max_inflight);
+/* Also hold back the next job while the trace elements (n_cols x 2^log_n) of the jobs in
+ * flight plus its own exceed max_trace_elements (0: no limit, the default); set before
+ * submitting.  No counterpart in the reference (it has no server). */
+int stwo_hip_server_limit_work(StwoProofServer *server, uint64_t max_trace_elements);
 int stwo_hip_server_submit(StwoProofServer *server, const StwoWideFibParams *params, uint64_t seed,
                            uint64_t *job_id);
 int stwo_hip_server_wait(StwoProofServer *server, uint32_t timeout_us, uint64_t *job_id, int *job_status,

@@ -9,7 +9,9 @@
 // once, which admits the next waiting job.  The twiddle trees (one per log size) stay
 // cached on the device for every later job.
 //
-// A job larger than the whole budget is admitted alone.  Results are returned in
+// A work limit (trace elements in flight, n_cols x 2^log_n per job) keeps large,
+// device-filling jobs from crowding each other while small ones still overlap.  A job
+// larger than the whole budget (or the work limit) is admitted alone.  Results are returned in
 // completion order with their submit / start / end times.
 #include <atomic>
 #include <chrono>
@@ -38,7 +40,7 @@ struct Running {
   Request r;
   std::unique_ptr<prover::Job> job;
   int slot;
-  uint64_t bytes;
+  uint64_t bytes, work;  // device footprint, trace elements
   double t_start, host_s;  // admission time, host time spent building and enqueuing the job
 };
 struct Result {
@@ -49,9 +51,13 @@ struct Result {
   std::string error;  // the worker's error message when status != OK
 };
 
+// trace elements of a job (n_cols x 2^log_n): its share of the device's compute
+inline uint64_t job_work(const prover::Params& p) { return (uint64_t)p.n_cols << p.log_n; }
+
 struct Server {
   uint64_t budget;
   uint32_t max_inflight;
+  std::atomic<uint64_t> work_limit{0};  // 0: none
   Clock::time_point t0 = Clock::now();
   std::vector<hipStream_t> streams;
   std::vector<bool> slot_busy;
@@ -64,7 +70,7 @@ struct Server {
   bool stopping = false;
 
   std::vector<Running> running;  // worker thread only
-  uint64_t inflight_bytes = 0;
+  uint64_t inflight_bytes = 0, inflight_work = 0;
   std::thread worker;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -87,6 +93,7 @@ struct Server {
     }
     slot_busy[r.slot] = false;
     inflight_bytes -= r.bytes;
+    inflight_work -= r.work;
     std::lock_guard<std::mutex> l(m);
     results.push_back(std::move(res));
     cv_done.notify_all();
@@ -113,13 +120,14 @@ struct Server {
           std::lock_guard<std::mutex> l(m);
           if (queue.empty() || running.size() >= max_inflight) break;
           const uint64_t need = prover::footprint_bytes(queue.front().p);
-          if (!running.empty() && inflight_bytes + need > budget) break;
+          const uint64_t w = job_work(queue.front().p), wl = work_limit.load();
+          if (!running.empty() && (inflight_bytes + need > budget || (wl && inflight_work + w > wl))) break;
           r = queue.front();
           queue.pop_front();
         }
         int slot = 0;
         while (slot_busy[slot]) slot++;
-        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), now(), 0.0};
+        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), job_work(r.p), now(), 0.0};
         int st = STWO_HIP_OK;
         {
           auto g = lock();
@@ -140,6 +148,7 @@ struct Server {
         }
         slot_busy[slot] = true;
         inflight_bytes += run.bytes;
+        inflight_work += run.work;
         running.push_back(std::move(run));
         progressed = true;
       }
@@ -213,6 +222,12 @@ extern "C" int stwo_hip_server_create(StwoProofServer** out, uint64_t hbm_budget
     });
     *out = h.release();
   });
+}
+
+extern "C" int stwo_hip_server_limit_work(StwoProofServer* h, uint64_t max_trace_elements) {
+  if (!h) return STWO_HIP_ERR_ARG;
+  h->s.work_limit.store(max_trace_elements);
+  return STWO_HIP_OK;
 }
 
 extern "C" int stwo_hip_server_submit(StwoProofServer* h, const StwoWideFibParams* params, uint64_t seed,

@@ -131,6 +131,7 @@ def _declare(L):
         "stwo_hip_wide_fib_proof_size": (C.c_int, [C.POINTER(WideFibParams), C.POINTER(u64), C.POINTER(u64)]),
         "stwo_hip_wide_fib_prove": (C.c_int, [C.POINTER(WideFibParams), u64, u32p, u64]),
         "stwo_hip_server_create": (C.c_int, [C.POINTER(vp), u64, u32]),
+        "stwo_hip_server_limit_work": (C.c_int, [vp, u64]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
         "stwo_hip_server_wait": (C.c_int, [vp, u32, C.POINTER(u64), C.POINTER(C.c_int), u32p, u64,
                                            C.POINTER(u64), C.POINTER(C.c_double)]),
@@ -749,10 +750,12 @@ def wide_fib_prove(params, seed):
 class ProofServer:
     """Continuous-batching proof server (stwo_hip_server_*)."""
 
-    def __init__(self, hbm_budget_bytes, max_inflight=4):
+    def __init__(self, hbm_budget_bytes, max_inflight=4, work_limit=0):
         h = vp()
         _c(lib().stwo_hip_server_create(C.byref(h), hbm_budget_bytes, max_inflight))
         self.handle = h.value
+        if work_limit:
+            _c(lib().stwo_hip_server_limit_work(self.handle, work_limit))
 
     def submit(self, params, seed):
         jid = u64()

@@ -1,6 +1,8 @@
 """GPU wide-Fibonacci prover: proofs equal the oracle-assembled proofs word for word,
 pass the verifier, and the proof server returns the same proof for a job alone and in
 a mix of concurrent jobs."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -107,6 +109,25 @@ def test_servers_back_to_back():
         srv.close()
 
 
+def test_server_admits_by_work_limit():
+    p = S.wide_fib_params(64, 12, 3, 4, 2)
+    work = 64 << 12
+    srv = S.ProofServer(8 << 30, 4, work + work // 2)  # one job's trace elements at a time
+    seeds = [17, 18, 19]
+    ids = {srv.submit(p, s): s for s in seeds}
+    spans, proofs = [], {}
+    for _ in seeds:
+        jid, st, proof, times = srv.wait(2_000_000)
+        assert st == 0
+        spans.append(times[1:3])
+        proofs[ids[jid]] = proof
+    srv.close()
+    spans.sort()
+    for a, b in zip(spans, spans[1:]):
+        assert b[0] >= a[1] - 1e-6, "two jobs ran together although the work limit admits one"
+    assert (proofs[17] == S.wide_fib_prove(p, 17)).all()
+
+
 @pytest.mark.parametrize("C,log_n,Q,pow_bits,last_log,seed", [(3, 4, 1, 0, 2, 1), (40, 6, 3, 6, 3, 2),
                                                               (33, 11, 3, 8, 2, 21), (64, 13, 2, 10, 4, 22)])
 def test_plain_path_matches_device_prover(C, log_n, Q, pow_bits, last_log, seed):
@@ -117,3 +138,30 @@ def test_plain_path_matches_device_prover(C, log_n, Q, pow_bits, last_log, seed)
     got = wide_fib_plain.prove(C, log_n, Q, pow_bits, last_log, seed)
     diff = np.nonzero(got != want)[0]
     assert diff.size == 0, f"first differing word {diff[:8]}"
+
+
+@pytest.mark.parametrize("C,log_n,Q,pow_bits,last_log", [
+    (2, 8, 3, 4, 2), (4097, 8, 3, 4, 2), (8, 3, 3, 4, 2), (8, 27, 3, 4, 2), (8, 8, 0, 4, 2), (8, 8, 65, 4, 2),
+    (8, 8, 3, 33, 2), (8, 8, 3, 4, 1), (8, 8, 3, 4, 11), (8, 4, 3, 4, 5)])
+def test_invalid_statements_rejected(C, log_n, Q, pow_bits, last_log):
+    """Out-of-range parameters are reported as errors before any device work, and the
+    library stays usable."""
+    p = S.wide_fib_params(C, log_n, Q, pow_bits, last_log)
+    with pytest.raises(S.StwoHipError):
+        S.wide_fib_proof_size(p)
+    with pytest.raises(S.StwoHipError):  # the prove entry point validates before its buffer
+        S.check(S.lib().stwo_hip_wide_fib_prove(ctypes.byref(p), 1, None, 0))
+    w = S.wide_fib_prove(S.wide_fib_params(3, 4, 1, 0, 2), 1)
+    V.verify(w)
+
+
+@pytest.mark.parametrize("slots", [0, 17])
+def test_server_slot_count_rejected(slots):
+    with pytest.raises(S.StwoHipError):
+        S.ProofServer(1 << 30, slots)
+
+
+def test_grind_bits_rejected():
+    with pytest.raises(S.StwoHipError):
+        S.grind(np.zeros(8, np.uint32), 41)
+    assert S.grind(np.zeros(8, np.uint32), 0) == 0
