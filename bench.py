@@ -172,7 +172,7 @@ def bench_lde_commit(args):
         "vs_baseline": None,
         "dtype": "u32 (M31)",
         "data": "synthetic (uniform M31, seeded counter-based stream)",
-        "config": {"workload": "cfg2 LDE+commit: 256 M31 columns log 20 -> LDE log 21 (blowup 2) + Blake2s Merkle",
+        "config": {"workload": "LDE+commit: 256 M31 columns log 20 -> LDE log 21 (blowup 2) + Blake2s Merkle",
                    "columns": C, "log_size": LOG_N, "log_blowup": LOG_BLOWUP, "parallelism": "single GPU",
                    "fft_schedule": {"contiguous_layers": schedule[0], "fused_strided": schedule[1]},
                    "launch": ("hipGraph replay of the commit plan" if plan.uses_graph() else
@@ -269,7 +269,7 @@ def _roof(bytes_, avg_ms, kernel, profile=None):
             "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms}
 
 
-def bench_cfg1(args):
+def bench_small(args):
     """16 columns of log 16: interpolate, evaluate on log 17, Merkle commit; root checked vs the oracle."""
     import numpy as np
     import stwo_hip as S
@@ -306,8 +306,8 @@ def bench_cfg1(args):
         extra["parity_root_matches_oracle"] = bool((oroot == tree.root()).all())
         extra["cpu_baseline"] = {"value": C * m / dt, "unit": "elems/s", "kind": "port", "sample": "whole config",
                                  "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
-    return _line("LDE+Merkle M31 elems/s (cfg1)", C * m / (ms * 1e-3), "elems/s", ms, args,
-                 {"workload": "cfg1: 16 M31 columns log 16 -> LDE log 17 + Blake2s Merkle"}, extra=extra)
+    return _line("LDE+Merkle M31 elems/s (small plain path)", C * m / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "small LDE+commit: 16 M31 columns log 16 -> LDE log 17 + Blake2s Merkle"}, extra=extra)
 
 
 def bench_batch_inverse(args):
@@ -334,7 +334,7 @@ def bench_batch_inverse(args):
                                  "cores": _cores()}
         extra["parity_matches_oracle"] = bool((dst.to_numpy().reshape(-1) == want.reshape(-1)).all())
     return _line("batch_inverse QM31 elems/s", n / (ms * 1e-3), "elems/s", ms, args,
-                 {"workload": "cfg2 batch_inverse: 2^24 QM31 (AoS)"},
+                 {"workload": "batch_inverse: 2^24 QM31 (AoS)"},
                  _roof(32 * n, avg, "batch_inverse_qm31", "batch_inverse_qm31_2e24"), extra)
 
 
@@ -385,7 +385,7 @@ def bench_fri(args):
                                  "sample": f"whole fold chain; {dt:.3f} s", "cores": _cores()}
         extra["parity_matches_oracle"] = bool(all((layers[i].to_numpy() == outs[i]).all() for i in range(len(layers))))
     return _line("FRI fold chain GB/s (log 22 circle -> line -> log 4)", total / (ms * 1e-3) / 1e9, "GB/s", ms, args,
-                 {"workload": "cfg2 FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
+                 {"workload": "FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
                  _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line",
                        "fri_fold_2e22_to_2e4"), extra)
 
@@ -430,12 +430,12 @@ def bench_quotients(args):
                                  "cores": _cores()}
         extra["parity_matches_oracle"] = bool((out.to_numpy() == want).all())
     return _line("accumulate_quotients rows/s", n / (ms * 1e-3), "rows/s", ms, args,
-                 {"workload": f"cfg3 quotients: {C} M31 columns log {L}, 3 QM31 points, {n_samples} samples"},
+                 {"workload": f"quotients: {C} M31 columns log {L}, 3 QM31 points, {n_samples} samples"},
                  _roof(alg, ks["accumulate_quotients"]["avg_ms"], "accumulate_quotients", "quotients_200x2e21_3pts"),
                  extra)
 
 
-def cfg4_log_sizes(n_cols=1200, seed=0x5EED0004):
+def mixed_log_sizes(n_cols=1200, seed=0x5EED0004):
     """log sizes drawn from {10..22} with weights proportional to 1/size."""
     import numpy as np
     ks = np.arange(10, 23)
@@ -452,7 +452,7 @@ def bench_batched(args):
     import numpy as np
     import stwo_hip as S
 
-    logs = cfg4_log_sizes()
+    logs = mixed_log_sizes()
     C = len(logs)
     L = S.lib()
     evals = []
@@ -546,9 +546,9 @@ def bench_batched(args):
         extra["cpu_baseline"] = {"value": elems / dt, "unit": "elems/s", "kind": "port", "sample": "whole config",
                                  "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
     plan.close()
-    return _line("LDE+Merkle M31 elems/s (cfg4 batched mixed sizes)", elems / (ms_graph * 1e-3), "elems/s",
+    return _line("LDE+Merkle M31 elems/s (batched mixed sizes)", elems / (ms_graph * 1e-3), "elems/s",
                  ms_graph, args,
-                 {"workload": f"cfg4: {C} M31 columns, log sizes 10..22 (weights 1/size), LDE blowup 2, one "
+                 {"workload": f"mixed commit: {C} M31 columns, log sizes 10..22 (weights 1/size), LDE blowup 2, one "
                               f"Blake2s tree with mixed-degree injection; {launch}"},
                  {"bound": "hbm", "kernel": "whole step (all kernels)", "achieved": round(gbs, 1),
                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
@@ -660,13 +660,16 @@ def bench_serving(args):
              "pow_bits": pow_bits, "last_layer_log": last_log, "rate_sweep": sweep}
     return _line("served wide-Fibonacci proofs/s (continuous batching)", batched["proofs_per_s"], "proofs/s",
                  1e3 / batched["proofs_per_s"], args,
-                 {"workload": f"cfg5 serving: {n_jobs} Poisson(rate {rate}/s) wide-Fibonacci prove jobs, n_cols in "
+                 {"workload": f"serving: {n_jobs} Poisson(rate {rate}/s) wide-Fibonacci prove jobs, n_cols in "
                               "{32,64,128,256,512}, log_n in 12..20, blowup 2, 3 queries, Blake2s channel"},
                  extra=extra)
 
 
-CONFIGS = {"cfg2": bench_lde_commit, "cfg1": bench_cfg1, "cfg2-binv": bench_batch_inverse, "cfg2-fri": bench_fri,
-           "cfg3": bench_quotients, "cfg4": bench_batched, "cfg5": bench_serving}
+CONFIGS = {"lde-commit": bench_lde_commit, "small": bench_small, "batch-inverse": bench_batch_inverse,
+           "fri": bench_fri, "quotients": bench_quotients, "mixed-commit": bench_batched, "serving": bench_serving}
+# short names used by earlier profiles and scripts
+CONFIGS.update({"cfg2": bench_lde_commit, "cfg1": bench_small, "cfg2-binv": bench_batch_inverse,
+                "cfg2-fri": bench_fri, "cfg3": bench_quotients, "cfg4": bench_batched, "cfg5": bench_serving})
 
 
 def main():
@@ -674,7 +677,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="lde-commit", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-sample-cols", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=120)
