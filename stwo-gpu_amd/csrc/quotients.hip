@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
 template <int NB>
 struct ColSlot {
   const uint32_t* col;  // the column (device pointer)
-  uint32_t pad[2];
+  uint32_t mask;        // bit b: batch b samples the column (alpha[b] may be nonzero)
+  uint32_t pad;
   qm31 alpha[NB];
 };
 
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
       const ColSlot<NB>& S = slots[h + i];
 #pragma unroll
       for (int b = 0; b < NB; b++) {
+        if (NB > 1 && !(S.mask & (1u << b))) continue;  // uniform: the slot is the same for every lane
         s[b][0] += (uint64_t)S.alpha[b].x.a * v[i];
         s[b][1] += (uint64_t)S.alpha[b].x.b * v[i];
         s[b][2] += (uint64_t)S.alpha[b].y.a * v[i];
@@ -184,9 +186,9 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* con
     const qm31 bb = qsub(qmul(v, c), qmul(a, py));
     la = qadd(la, qmul(alpha, a));
     lb = qadd(lb, qmul(alpha, bb));
-    ss[j] = ColSlot<1>{cols[j], {0, 0}, {alpha}};
+    ss[j] = ColSlot<1>{cols[j], 1, 0, {alpha}};
   }
-  for (uint32_t j = count + threadIdx.x; j < n_slots; j += SB_NT) ss[j] = ColSlot<1>{cols[0], {0, 0}, {qzero()}};
+  for (uint32_t j = count + threadIdx.x; j < n_slots; j += SB_NT) ss[j] = ColSlot<1>{cols[0], 0, 0, {qzero()}};
   sa[threadIdx.x] = la;
   sb[threadIdx.x] = lb;
   __syncthreads();
@@ -249,6 +251,7 @@ void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t*
         it = by_col.emplace(S.col, z).first;
       }
       it->second.alpha[b] = qadd(it->second.alpha[b], S.alpha);
+      it->second.mask |= 1u << b;
     }
   std::vector<quot::ColSlot<NB>> slots;
   for (auto& kv : by_col) slots.push_back(kv.second);
