@@ -46,7 +46,7 @@ void wide_fib_quotients_async(uint32_t* const q[4], const uint32_t* const* d_tra
 size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L);
 void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
                                  qm31* d_out, void* d_scratch);
-size_t quotient_single_point_scratch_bytes(uint32_t n_cols);
+size_t quotient_single_point_scratch_bytes(uint32_t n_cols, uint32_t max_log);
 void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
                                              const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
                                              const qm31* d_values, const qm31* d_rc, void* d_scratch,
@@ -290,7 +290,7 @@ struct Arena {
     for (uint32_t j = 1; j <= L.n_line; j++) tree_l.push_back(take(32 * ((2ull << (p.log_n + 2 - j)) - 1)));
     eap = take(std::max(eval_at_point_columns_scratch_bytes(p.n_cols, p.log_n),
                         eval_at_point_columns_scratch_bytes(4, p.log_n + 1)));
-    quot = take(quotient_single_point_scratch_bytes(std::max<uint32_t>(p.n_cols, 4)));
+    quot = take(quotient_single_point_scratch_bytes(std::max<uint32_t>(p.n_cols, 4), p.log_n + 2));
     samples = take(16 * (C + 4));
     jb = take(sizeof(JobBlock));
     proof = take(4 * L.total);

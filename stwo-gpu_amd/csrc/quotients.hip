@@ -41,7 +41,37 @@ struct Batch {
 
 __device__ __forceinline__ uint64_t fold64(uint64_t s) { return (s & P) + (s >> 31); }
 
-__global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
+// Points of CircleDomain(half coset (hi, hs) of log half_log) from two small tables
+// instead of a 31-step power per row: lo[j] = G^(hi + hs j) for j < 2^k and
+// up[j] = G^(hs j 2^k) for j < 2^(half_log - k), so position i < 2^half_log is
+// lo[i mod 2^k] up[i >> k] and position i + 2^half_log its conjugate
+// (CircleDomain::index_at, cuda/src/quotients.cu:17-29).
+struct DomainTables {
+  const pt* lo;
+  const pt* up;
+  uint32_t k, half_log;
+};
+constexpr uint32_t DT_LO_BITS = 11;
+inline uint32_t dt_k(uint32_t half_log) { return half_log < DT_LO_BITS ? half_log : DT_LO_BITS; }
+inline size_t dt_bytes(uint32_t half_log) {
+  const uint32_t k = dt_k(half_log);
+  return (sizeof(pt) * ((1ull << k) + (1ull << (half_log - k))) + 255) & ~(size_t)255;
+}
+__global__ void domain_tables_kernel(uint32_t hi, uint32_t hs, uint32_t half_log, uint32_t k, pt* lo, pt* up) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (1u << k)) lo[t] = point_at_index((uint32_t)((hi + (uint64_t)hs * t) & 0x7fffffffu));
+  if (t < (1u << (half_log - k))) up[t] = point_at_index((uint32_t)(((uint64_t)hs * ((uint64_t)t << k)) & 0x7fffffffu));
+}
+__device__ __forceinline__ pt domain_point(const DomainTables& T, uint32_t i) {
+  const uint32_t half = 1u << T.half_log;
+  const bool conj = i >= half;
+  const uint32_t j = conj ? i - half : i;
+  pt q = pmul(T.lo[j & ((1u << T.k) - 1)], T.up[j >> T.k]);
+  if (conj) q.y = neg(q.y);
+  return q;
+}
+
+__global__ __launch_bounds__(256) void accumulate_kernel(DomainTables T, uint32_t log_size,
                                                          const uint32_t* const* __restrict__ cols,
                                                          const Sample* __restrict__ samples,
                                                          const Batch* __restrict__ batches, uint32_t n_batches,
@@ -49,7 +79,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(uint32_t hi, uint32_t h
                                                          uint32_t* __restrict__ o2, uint32_t* __restrict__ o3) {
   uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= (1u << log_size)) return;
-  pt p = point_at_index(circle_domain_index(hi, hs, log_size - 1, brev(row, log_size)));
+  pt p = domain_point(T, brev(row, log_size));
   qm31 acc = qzero();
   for (uint32_t b = 0; b < n_batches; b++) {
     const Batch B = batches[b];
@@ -103,17 +133,19 @@ struct ColSlot {
   qm31 alpha[NB];
 };
 
-__device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)[4], pt p, qm31 acc) {
+__device__ __forceinline__ cm31 batch_denominator(const Batch& B, pt p) {
+  return csub(cmul(cm31{sub(B.prx.a, p.x), B.prx.b}, B.piy), cmul(cm31{sub(B.pry.a, p.y), B.pry.b}, B.pix));
+}
+__device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)[4], pt p, cm31 dinv, qm31 acc) {
   qm31 sum = {{reduce63(s[0]), reduce63(s[1])}, {reduce63(s[2]), reduce63(s[3])}};
   qm31 num = qsub(qmul(B.c, sum), qadd(qscale(B.a_sum, p.y), B.b_sum));
-  cm31 d = csub(cmul(cm31{sub(B.prx.a, p.x), B.prx.b}, B.piy), cmul(cm31{sub(B.pry.a, p.y), B.pry.b}, B.pix));
-  return qadd(qmul(acc, B.coeff), qmul_cm(num, cinv(d)));
+  return qadd(qmul(acc, B.coeff), qmul_cm(num, dinv));
 }
 
 constexpr int QG = 6;  // columns per load group (a multiple of the fold period 3)
 
 template <int NB>
-__global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint32_t hs, uint32_t log_size,
+__global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, uint32_t log_size,
                                                               const ColSlot<NB>* __restrict__ slots,
                                                               uint32_t n_slots, const Batch* __restrict__ batches,
                                                               uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
@@ -156,10 +188,25 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(uint32_t hi, uint3
 #pragma unroll
     for (int i = 0; i < QG; i++) v[i] = vn[i];
   }
-  pt p = point_at_index(circle_domain_index(hi, hs, log_size - 1, brev(row, log_size)));
+  const pt p = domain_point(T, brev(row, log_size));
+  // one CM31 inversion for the NB denominators (prefix products, Montgomery's trick)
+  cm31 d[NB], pre[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    d[b] = batch_denominator(batches[b], p);
+    pre[b] = b == 0 ? d[0] : cmul(pre[b - 1], d[b]);
+  }
+  cm31 inv_all = cinv(pre[NB - 1]);
+  cm31 dinv[NB];
+#pragma unroll
+  for (int b = NB - 1; b > 0; b--) {
+    dinv[b] = cmul(inv_all, pre[b - 1]);
+    inv_all = cmul(inv_all, d[b]);
+  }
+  dinv[0] = inv_all;
   qm31 acc = qzero();
 #pragma unroll
-  for (int b = 0; b < NB; b++) acc = finish_batch(batches[b], s[b], p, acc);
+  for (int b = 0; b < NB; b++) acc = finish_batch(batches[b], s[b], p, dinv[b], acc);
   o0[row] = acc.x.a;
   o1[row] = acc.x.b;
   o2[row] = acc.y.a;
@@ -209,8 +256,24 @@ static uint32_t padded_slots(uint32_t n_cols) { return (n_cols + quot::QG - 1) /
 static size_t slot_bytes(uint32_t n_slots) {
   return (sizeof(quot::ColSlot<1>) * (n_slots + quot::QG) + 255) & ~(size_t)255;
 }
-size_t quotient_single_point_scratch_bytes(uint32_t n_cols) {
-  return slot_bytes(padded_slots(std::max<uint32_t>(n_cols, 1))) + sizeof(quot::Batch);
+static constexpr size_t BATCH_BYTES = (sizeof(quot::Batch) + 255) & ~(size_t)255;
+// scratch of accumulate_quotients_single_point_async for domains up to log max_log:
+// slots | batch | domain tables
+size_t quotient_single_point_scratch_bytes(uint32_t n_cols, uint32_t max_log) {
+  return slot_bytes(padded_slots(std::max<uint32_t>(n_cols, 1))) + BATCH_BYTES + quot::dt_bytes(max_log - 1);
+}
+
+// Fill the domain tables of CircleDomain(half coset (hi, hs), log log_size) at `dev`
+// (dt_bytes(log_size - 1) bytes) on the current stream.
+static quot::DomainTables domain_tables_async(uint32_t hi, uint32_t hs, uint32_t log_size, void* dev) {
+  const uint32_t hl = log_size - 1, k = quot::dt_k(hl);
+  pt* lo = (pt*)dev;
+  pt* up = lo + (1u << k);
+  const uint32_t n = std::max(1u << k, 1u << (hl - k));
+  timed("quotient_domain_tables", [&] {
+    quot::domain_tables_kernel<<<ceil_div(n, 256), 256, 0, ctx().cur>>>(hi, hs, hl, k, lo, up);
+  });
+  return quot::DomainTables{lo, up, k, hl};
 }
 
 // accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
@@ -223,13 +286,14 @@ void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t 
   const uint32_t n_slots = padded_slots(n_cols);
   quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
   quot::Batch* bs = (quot::Batch*)((char*)d_scratch + slot_bytes(n_slots));
+  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, (char*)bs + BATCH_BYTES);
   timed("quotient_constants", [&] {
     quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss,
                                                                 n_slots + quot::QG, bs);
   });
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<1><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        hi, hs, log_size, ss, n_slots, bs, out[0], out[1], out[2], out[3]);
+        T, log_size, ss, n_slots, bs, out[0], out[1], out[2], out[3]);
   });
 }
 
@@ -263,9 +327,11 @@ void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t*
   for (int i = 0; i < quot::QG; i++) slots.push_back(pad);  // read by the last group's prefetch
   DevBuf dsl = upload(slots.data(), slots.size() * sizeof(quot::ColSlot<NB>));
   DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
+  DevBuf dt(quot::dt_bytes(log_size - 1));
+  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, dt.p);
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<NB><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        hi, hs, log_size, dsl.as<quot::ColSlot<NB>>(), n_loop, dbs.as<quot::Batch>(), out[0], out[1], out[2], out[3]);
+        T, log_size, dsl.as<quot::ColSlot<NB>>(), n_loop, dbs.as<quot::Batch>(), out[0], out[1], out[2], out[3]);
   });
 }
 void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,
@@ -328,9 +394,11 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
   if (bs.empty()) bs.push_back(quot::Batch{});
   DevBuf dss = upload(ss.data(), ss.size() * sizeof(quot::Sample));
   DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
+  DevBuf dt(quot::dt_bytes(log_size - 1));
+  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, dt.p);
   timed("accumulate_quotients", [&] {
     quot::accumulate_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().cur>>>(
-        hi, hs, log_size, d_cols, dss.as<quot::Sample>(), dbs.as<quot::Batch>(), n_batches, out[0], out[1], out[2],
+        T, log_size, d_cols, dss.as<quot::Sample>(), dbs.as<quot::Batch>(), n_batches, out[0], out[1], out[2],
         out[3]);
   });
 }
