@@ -15,8 +15,15 @@
 //     as [t low index bits | k layer bits | hb high bits]; the t low bits are
 //     contiguous in HBM, so every access moves >= 128-byte runs.
 //   * Within a pass the layers are done radix-8 in registers; between radix
-//     stages the tile is exchanged through LDS.
-//   * interpolate: the 2^-n scale is fused into the store of the last pass.
+//     stages the tile is exchanged through LDS.  The next stage's twiddles are
+//     loaded into registers before each exchange, so their latency overlaps the
+//     barriers.
+//   * Butterflies multiply by twice the twiddle (mul_dbl in field.hpp), one
+//     instruction fewer per product mod P than a plain multiplication; the passes
+//     are VALU-bound (about 12 instructions per butterfly).
+//   * Global accesses use a scalar column base and a 32-bit vector offset.
+//   * interpolate: the 2^-n scale is fused into the store of the last pass (a
+//     31-bit rotation, since 2^-n = 2^(31-n) mod P).
 //   * evaluate from coefficients (zero padded to the domain): the top layers on
 //     zeros only duplicate values, so the first pass reads coeffs[idx mod 2^c]
 //     and skips those layers (no extend copy, no wasted butterflies).
