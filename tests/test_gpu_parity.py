@@ -189,6 +189,24 @@ def test_evaluate_columns_blowup_4():
         assert (c.to_numpy() == want).all()
 
 
+@pytest.mark.parametrize("log_n,n_cols", [(15, 11), (19, 9)])
+def test_interpolate_evaluate_columns_partial_band(log_n, n_cols):
+    """Separate contiguous and strided passes over distinct columns whose count leaves a
+    partial band of 8 columns."""
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(coset)
+    itw = tree.itwiddles.to_numpy()
+    evals = [rand_m31(1 << log_n, 9100 + c) for c in range(n_cols)]
+    cols = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    S.interpolate_columns(cols, tree)
+    want = [O.interpolate(e, coset.initial_index, itw) for e in evals]
+    for c in range(n_cols):
+        assert (cols[c].to_numpy() == want[c]).all(), f"interpolate column {c}"
+    S.evaluate_columns(cols, tree)
+    for c in range(n_cols):
+        assert (cols[c].to_numpy() == evals[c]).all(), f"evaluate column {c}"
+
+
 @pytest.mark.parametrize("log_n,blow", [(3, 1), (11, 1), (12, 1), (13, 1), (16, 1), (16, 2), (17, 3)])
 def test_lde_async_matches_oracle(log_n, blow):
     """Interpolate + evaluate-from-coefficients (no extend copy) on several columns."""
@@ -408,7 +426,10 @@ def test_errors_are_reported_not_fatal():
 # ------------------------------------------------------------------ one-call LDE (fused middle pass)
 @pytest.mark.parametrize("log_n,blow,n_cols,in_place", [(3, 1, 3, False), (13, 1, 2, False), (14, 1, 3, True),
                                                          (15, 2, 2, False), (17, 3, 2, True), (20, 1, 2, False),
-                                                         (21, 1, 1, True), (22, 1, 1, False)])
+                                                         (21, 1, 1, True), (22, 1, 1, False),
+                                                         # column counts that leave a partial band of 8
+                                                         (14, 1, 11, False), (17, 1, 10, True), (16, 1, 19, False),
+                                                         (18, 1, 9, True)])
 def test_lde_columns_fused(log_n, blow, n_cols, in_place):
     m = log_n + blow
     root = S.canonic_half_coset(m)
