@@ -227,6 +227,18 @@ int stwo_hip_evaluate_columns_async(const uint32_t *const *coeffs, const uint32_
                                     uint32_t *const *dst, const uint32_t *eval_log, uint32_t n_cols,
                                     const uint32_t *twiddles, uint64_t tree_len);
 
+/* Column::at and decommitment gathers (replace the whole-column D2H copy per element of
+ * column.rs:49-52 and the per-query host walks of MerkleProver::decommit).
+ * gather_rows: values[i * n_cols + c] = columns[c][rows[i]]; columns: HOST array of device
+ * columns of column_size values each; rows, values: HOST.  merkle_auth_paths: layers: HOST
+ * array of max_log + 1 device layers (layers[k] holds 2^k hashes, k = max_log the leaves);
+ * paths[q * max_log + j] = sibling at layer max_log - j of the node on the path of leaf
+ * leaves[q]; leaves, paths: HOST.  Both synchronous; indices are checked on the host. */
+int stwo_hip_gather_rows(const uint32_t *const *columns, uint32_t n_cols, uint64_t column_size, const uint32_t *rows,
+                         uint32_t n_rows, uint32_t *values);
+int stwo_hip_merkle_auth_paths(StwoBlake2sHash *const *layers, uint32_t max_log, const uint32_t *leaves,
+                               uint32_t n_leaves, StwoBlake2sHash *paths);
+
 /* Whole Merkle tree (MerkleProver::commit semantics): columns sorted by decreasing size
  * (stable), one layer per log size from the largest down to 0, columns of each size injected
  * into that layer.  layers: HOST array of (max_log + 1) device buffers, layers[k] holding

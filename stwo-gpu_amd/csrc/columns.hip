@@ -88,10 +88,80 @@ __global__ void accumulate_kernel(uint32_t n, Soa4 l, Soa4 r) {
   for (int k = 0; k < 4; k++) l.c[k][i] = add(l.c[k][i], r.c[k][i]);
 }
 
+// ---- gathers for Column::at and Merkle decommitment ----
+// values[i * n_cols + c] = cols[c][rows[i]]: every queried row of every column in one
+// launch (the reference copies a whole column to the host per element, column.rs:49-52).
+__global__ __launch_bounds__(256) void gather_rows_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                          const uint32_t* __restrict__ rows, uint64_t total,
+                                                          uint32_t* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t i = (uint32_t)(t / n_cols), c = (uint32_t)(t % n_cols);
+  out[t] = cols[c][rows[i]];
+}
+struct Hash8 {
+  uint32_t s[8];
+};
+// paths[q * depth + j] = the sibling, at layer depth - j, of the node on leaf q's path
+// (j = 0 is the leaf's own sibling): the authentication path of MerkleProver::decommit.
+__global__ __launch_bounds__(256) void auth_paths_kernel(const Hash8* const* __restrict__ layers, uint32_t depth,
+                                                         const uint32_t* __restrict__ leaves, uint32_t total,
+                                                         Hash8* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t q = t / depth, j = t % depth;
+  out[t] = layers[depth - j][(leaves[q] >> j) ^ 1u];
+}
+
 }  // namespace cols
 }  // namespace sh
 
 using namespace sh;
+
+extern "C" int stwo_hip_gather_rows(const uint32_t* const* columns, uint32_t n_cols, uint64_t column_size,
+                                    const uint32_t* rows, uint32_t n_rows, uint32_t* values) {
+  return guarded([&] {
+    if (!n_cols || !n_rows) return;
+    require(columns != nullptr && rows != nullptr && values != nullptr, "gather_rows: null argument");
+    for (uint32_t i = 0; i < n_rows; i++) require(rows[i] < column_size, "gather_rows: row index out of range");
+    const uint64_t total = (uint64_t)n_rows * n_cols;
+    require(total <= (1ull << 31), "gather_rows: too many values");
+    DevBuf table = upload(columns, n_cols * sizeof(void*));
+    DevBuf drows = upload(rows, n_rows * sizeof(uint32_t));
+    DevBuf out(total * sizeof(uint32_t));
+    timed("gather_rows", [&] {
+      cols::gather_rows_kernel<<<ceil_div(total, 256), 256, 0, ctx().cur>>>(table.as<const uint32_t*>(), n_cols,
+                                                                            drows.as<uint32_t>(), total,
+                                                                            out.as<uint32_t>());
+    });
+    hip_check(hipMemcpyAsync(values, out.p, total * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx().cur),
+              "hipMemcpyAsync D2H");
+    sync();
+  });
+}
+
+extern "C" int stwo_hip_merkle_auth_paths(StwoBlake2sHash* const* layers, uint32_t max_log, const uint32_t* leaves,
+                                          uint32_t n_leaves, StwoBlake2sHash* paths) {
+  return guarded([&] {
+    if (!n_leaves || !max_log) return;
+    require(layers != nullptr && leaves != nullptr && paths != nullptr, "merkle_auth_paths: null argument");
+    require(max_log <= 31, "merkle_auth_paths: at most 31 layers");
+    for (uint32_t q = 0; q < n_leaves; q++)
+      require(leaves[q] < (1u << max_log), "merkle_auth_paths: leaf index out of range");
+    const uint64_t total = (uint64_t)n_leaves * max_log;
+    require(total < (1ull << 31), "merkle_auth_paths: too many hashes");
+    DevBuf table = upload(layers, (max_log + 1) * sizeof(void*));
+    DevBuf dleaves = upload(leaves, n_leaves * sizeof(uint32_t));
+    DevBuf out(total * sizeof(StwoBlake2sHash));
+    timed("merkle_auth_paths", [&] {
+      cols::auth_paths_kernel<<<ceil_div(total, 256), 256, 0, ctx().cur>>>(
+          table.as<const cols::Hash8*>(), max_log, dleaves.as<uint32_t>(), (uint32_t)total, out.as<cols::Hash8>());
+    });
+    hip_check(hipMemcpyAsync(paths, out.p, total * sizeof(StwoBlake2sHash), hipMemcpyDeviceToHost, ctx().cur),
+              "hipMemcpyAsync D2H");
+    sync();
+  });
+}
 
 extern "C" void bit_reverse_base_field(uint32_t* array, size_t size) {
   guarded([&] {

@@ -132,6 +132,8 @@ def _declare(L):
         "stwo_hip_wide_fib_prove": (C.c_int, [C.POINTER(WideFibParams), u64, u32p, u64]),
         "stwo_hip_server_create": (C.c_int, [C.POINTER(vp), u64, u32]),
         "stwo_hip_server_limit_work": (C.c_int, [vp, u64]),
+        "stwo_hip_gather_rows": (C.c_int, [vp, u32, u64, u32p, u32, u32p]),
+        "stwo_hip_merkle_auth_paths": (C.c_int, [vp, u32, u32p, u32, u32p]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
         "stwo_hip_server_wait": (C.c_int, [vp, u32, C.POINTER(u64), C.POINTER(C.c_int), u32p, u64,
                                            C.POINTER(u64), C.POINTER(C.c_double)]),
@@ -721,6 +723,26 @@ def eval_at_point_columns(cols, px, py):
     _c(lib().stwo_hip_eval_at_point_columns(_ptr_array([c.ptr for c in cols]), len(cols), log, qm31(px), qm31(py),
                                             C.cast(out, vp)))
     return [from_qm31(v) for v in out]
+
+
+def gather_rows(cols, rows):
+    """values[i, c] = cols[c][rows[i]] (Column::at for many rows and columns at once)."""
+    r = np.ascontiguousarray(rows, np.uint32)
+    out = np.zeros((r.size, len(cols)), np.uint32)
+    _c(lib().stwo_hip_gather_rows(_ptr_array([c.ptr for c in cols]), len(cols), cols[0].size if cols else 0,
+                                  r.ctypes.data_as(u32p), r.size, out.ctypes.data_as(u32p)))
+    return out
+
+
+def merkle_auth_paths(layers, leaves):
+    """paths[q, j] = the sibling hash (8 words) at layer max_log - j on the path of leaf leaves[q];
+    layers[k] holds the 2^k hashes of layer k."""
+    q = np.ascontiguousarray(leaves, np.uint32)
+    max_log = len(layers) - 1
+    out = np.zeros((q.size, max_log, 8), np.uint32)
+    _c(lib().stwo_hip_merkle_auth_paths(_ptr_array([l.ptr for l in layers]), max_log, q.ctypes.data_as(u32p), q.size,
+                                        out.ctypes.data_as(u32p)))
+    return out
 
 
 def grind(digest, pow_bits):

@@ -533,3 +533,30 @@ def test_eval_at_point_columns(log_n, n_cols):
     got = S.eval_at_point_columns([S.BaseFieldVec.from_numpy(c) for c in cols], px, py)
     for c in range(n_cols):
         assert tuple(got[c]) == O.eval_at_point(cols[c], px, py), f"column {c}"
+
+
+# ------------------------------------------------------------------ Column::at / decommitment gathers
+def test_gather_rows_and_auth_paths():
+    sizes = [12, 12, 9, 9, 9, 5]
+    cols_np = [rand_m31(1 << l, 9500 + i) for i, l in enumerate(sizes)]
+    cols = [S.BaseFieldVec.from_numpy(c) for c in cols_np]
+    rows = np.array([0, 4095, 17, 17, 2048], np.uint32)
+    got = S.gather_rows(cols[:2], rows)
+    want = np.stack([cols_np[0][rows], cols_np[1][rows]], axis=1)
+    assert (got == want).all()
+    with pytest.raises(S.StwoHipError):
+        S.gather_rows(cols[2:5], np.array([1, 512], np.uint32))  # row 512 is past a column of 2^9
+    tree = S.MerkleTree(cols, sizes)
+    tree.commit_async()
+    S.synchronize()
+    layers = [tree.layers[k].to_numpy().reshape(-1, 8) for k in range(len(tree.layers))]
+    want_layers = O.merkle_commit(cols_np)
+    assert all((layers[k] == want_layers[k].reshape(-1, 8)).all() for k in range(len(layers)))
+    leaves = np.array([0, 1, 4095, 1234, 3000], np.uint32)
+    paths = S.merkle_auth_paths(tree.layers, leaves)
+    max_log = len(layers) - 1
+    for qi, leaf in enumerate(leaves):
+        for j in range(max_log):
+            assert (paths[qi, j] == layers[max_log - j][(int(leaf) >> j) ^ 1]).all()
+    with pytest.raises(S.StwoHipError):
+        S.merkle_auth_paths(tree.layers, np.array([1 << max_log], np.uint32))
