@@ -293,6 +293,10 @@ int stwo_hip_lde_schedule(uint32_t log_size, uint32_t log_blowup, uint64_t n_col
  * 16-byte aligned columns are required there. */
 int stwo_hip_eval_at_point_columns(const uint32_t *const *columns, uint32_t n_cols, uint32_t log_size,
                                    StwoQM31 point_x, StwoQM31 point_y, StwoQM31 *values);
+/* The same for several points: values[k * n_cols + c] = column c at points[k] (the batched
+ * OODS evaluation over (column, point) pairs; points: HOST). */
+int stwo_hip_eval_at_points_columns(const uint32_t *const *columns, uint32_t n_cols, uint32_t log_size,
+                                    const StwoCirclePointQM31 *points, uint32_t n_points, StwoQM31 *values);
 
 /* Proof of work (GrindOps<Blake2sChannel>::grind, which the reference delegates to the CPU
  * SimdBackend, stwo_gpu_backend/src/backend.rs:14-18): the smallest nonce whose

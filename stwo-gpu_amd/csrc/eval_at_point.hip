@@ -308,21 +308,36 @@ extern "C" StwoQM31 eval_at_point(const uint32_t* coeffs, uint32_t coeffs_size, 
   return out;
 }
 
-extern "C" int stwo_hip_eval_at_point_columns(const uint32_t* const* columns, uint32_t n_cols, uint32_t log_size,
-                                              StwoQM31 point_x, StwoQM31 point_y, StwoQM31* values) {
+extern "C" int stwo_hip_eval_at_points_columns(const uint32_t* const* columns, uint32_t n_cols, uint32_t log_size,
+                                               const StwoCirclePointQM31* points, uint32_t n_points,
+                                               StwoQM31* values) {
   return guarded([&] {
-    require(columns != nullptr && values != nullptr && n_cols >= 1 && log_size >= 1 && log_size <= 30,
-            "eval_at_point_columns: bad arguments");
-    const qm31 pt[2] = {qm31{{point_x.a, point_x.b}, {point_x.c, point_x.d}},
-                        qm31{{point_y.a, point_y.b}, {point_y.c, point_y.d}}};
+    require(columns != nullptr && values != nullptr && points != nullptr && n_cols >= 1 && n_points >= 1 &&
+                log_size >= 1 && log_size <= 30,
+            "eval_at_points_columns: bad arguments");
+    std::vector<qm31> pts(2 * (size_t)n_points);
+    for (uint32_t k = 0; k < n_points; k++) {
+      const StwoCirclePointQM31& q = points[k];
+      pts[2 * k] = qm31{{q.x.a, q.x.b}, {q.x.c, q.x.d}};
+      pts[2 * k + 1] = qm31{{q.y.a, q.y.b}, {q.y.c, q.y.d}};
+    }
     DevBuf table = upload(columns, (size_t)n_cols * sizeof(void*));
-    DevBuf dpt = upload(pt, sizeof(pt));
-    DevBuf out(sizeof(qm31) * n_cols);
+    DevBuf dpt = upload(pts.data(), pts.size() * sizeof(qm31));
+    DevBuf out(sizeof(qm31) * n_cols * (size_t)n_points);
     DevBuf scratch(eval_at_point_columns_scratch_bytes(n_cols, log_size));
-    eval_at_point_columns_async(table.as<const uint32_t*>(), n_cols, log_size, dpt.as<qm31>(), out.as<qm31>(),
-                                scratch.p);
-    hip_check(hipMemcpyAsync(values, out.p, sizeof(qm31) * n_cols, hipMemcpyDeviceToHost, ctx().cur),
+    // one batched evaluation per point (the scratch is reused in stream order)
+    for (uint32_t k = 0; k < n_points; k++)
+      eval_at_point_columns_async(table.as<const uint32_t*>(), n_cols, log_size, dpt.as<qm31>() + 2 * k,
+                                  out.as<qm31>() + (size_t)k * n_cols, scratch.p);
+    hip_check(hipMemcpyAsync(values, out.p, sizeof(qm31) * n_cols * (size_t)n_points, hipMemcpyDeviceToHost,
+                             ctx().cur),
               "hipMemcpyAsync D2H");
     sync();
   });
+}
+
+extern "C" int stwo_hip_eval_at_point_columns(const uint32_t* const* columns, uint32_t n_cols, uint32_t log_size,
+                                              StwoQM31 point_x, StwoQM31 point_y, StwoQM31* values) {
+  const StwoCirclePointQM31 p{point_x, point_y};
+  return stwo_hip_eval_at_points_columns(columns, n_cols, log_size, &p, 1, values);
 }

@@ -535,6 +535,17 @@ def test_eval_at_point_columns(log_n, n_cols):
         assert tuple(got[c]) == O.eval_at_point(cols[c], px, py), f"column {c}"
 
 
+@pytest.mark.parametrize("log_n,n_cols", [(3, 2), (14, 5)])
+def test_eval_at_points_columns(log_n, n_cols):
+    rng = random.Random(190 + log_n)
+    points = [_point(rng) for _ in range(3)]
+    cols = [rand_m31(1 << log_n, 9300 + 17 * log_n + c) for c in range(n_cols)]
+    got = S.eval_at_points_columns([S.BaseFieldVec.from_numpy(c) for c in cols], points)
+    for k, (px, py) in enumerate(points):
+        for c in range(n_cols):
+            assert tuple(got[k][c]) == O.eval_at_point(cols[c], px, py), f"point {k} column {c}"
+
+
 # ------------------------------------------------------------------ Column::at / decommitment gathers
 def test_gather_rows_and_auth_paths():
     sizes = [12, 12, 9, 9, 9, 5]
