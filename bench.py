@@ -667,9 +667,6 @@ def bench_serving(args):
 
 CONFIGS = {"lde-commit": bench_lde_commit, "small": bench_small, "batch-inverse": bench_batch_inverse,
            "fri": bench_fri, "quotients": bench_quotients, "mixed-commit": bench_batched, "serving": bench_serving}
-# short names used by earlier profiles and scripts
-CONFIGS.update({"cfg2": bench_lde_commit, "cfg1": bench_small, "cfg2-binv": bench_batch_inverse,
-                "cfg2-fri": bench_fri, "cfg3": bench_quotients, "cfg4": bench_batched, "cfg5": bench_serving})
 
 
 def main():
