@@ -123,8 +123,9 @@ __global__ __launch_bounds__(256) void accumulate_kernel(DomainTables T, uint32_
 // once and multiplied into the sums of all NB batches, with alpha = 0 for a batch that
 // does not sample the column.  Per column the host provides NB alphas (one uniform
 // 16 * NB-byte record), so the loop has no data-dependent branches; loads run one group
-// of QG columns ahead of the arithmetic.  Sums are folded after every third column
-// (three products < 2^64 on top of a folded sum).
+// of QG columns ahead of the arithmetic.  Sums are folded after every fourth column:
+// a product of two values below P is at most (P - 1)^2 = 2^62 - 2^33 + 4, and a folded
+// sum is below 2^34, so four products on top of it stay below 2^64.
 template <int NB>
 struct ColSlot {
   const uint32_t* col;  // the column (device pointer)
@@ -142,7 +143,7 @@ __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)
   return qadd(qmul(acc, B.coeff), qmul_cm(num, dinv));
 }
 
-constexpr int QG = 6;  // columns per load group (a multiple of the fold period 3)
+constexpr int QG = 8;  // columns per load group (a multiple of the fold period 4)
 
 template <int NB>
 __global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, uint32_t log_size,
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, ui
         s[b][2] += (uint64_t)S.alpha[b].y.a * v[i];
         s[b][3] += (uint64_t)S.alpha[b].y.b * v[i];
       }
-      if (i % 3 == 2) {
+      if (i % 4 == 3) {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           s[b][0] = fold64(s[b][0]);
