@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "field.hpp"
+#include "gmem.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -148,12 +149,22 @@ __global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __rest
   qm31 l[16];
 #pragma unroll
   for (int j = 0; j < 16; j++) l[j] = lo[16 * t + j];
-  const size_t off = ((size_t)h << LO_BITS) + 16 * t;
-  for (uint32_t g = 0; g < GEMV_COLS && c0 + g < n_cols; g++) {
-    const uint4* p = reinterpret_cast<const uint4*>(cols[c0 + g] + off);
-    const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-    const uint32_t v[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                            q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+  // the 16 coefficients of column c0 + g for this lane: global loads (not flat, so waiting
+  // for the wave shuffles does not wait for them), the next column's issued before the
+  // current one is reduced; the column index is clamped so the prefetch needs no branch
+  const uint32_t off4 = (uint32_t)((((size_t)h << LO_BITS) + 16 * t) >> 2);
+  const uint32_t n_here = min(GEMV_COLS, n_cols - c0);
+  auto load16 = [&](uint32_t g, uint4 (&q)[4]) {
+    const SH_GLOBAL uint4* p = reinterpret_cast<const SH_GLOBAL uint4*>(as_global(cols[c0 + min(g, n_here - 1)]));
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = at(p, off4 + k);
+  };
+  uint4 qc[4], qn[4];
+  load16(0, qc);
+  for (uint32_t g = 0; g < n_here; g++) {
+    load16(g + 1, qn);
+    const uint32_t v[16] = {qc[0].x, qc[0].y, qc[0].z, qc[0].w, qc[1].x, qc[1].y, qc[1].z, qc[1].w,
+                            qc[2].x, qc[2].y, qc[2].z, qc[2].w, qc[3].x, qc[3].y, qc[3].z, qc[3].w};
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
@@ -161,13 +172,15 @@ __global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __rest
       s1 += (uint64_t)l[j].x.b * v[j];
       s2 += (uint64_t)l[j].y.a * v[j];
       s3 += (uint64_t)l[j].y.b * v[j];
-      if (j % 3 == 2) {  // at most three products between folds
+      if (j % 4 == 3) {  // four products of canonical values fit on top of a folded sum
         s0 = (s0 & P) + (s0 >> 31);
         s1 = (s1 & P) + (s1 >> 31);
         s2 = (s2 & P) + (s2 >> 31);
         s3 = (s3 & P) + (s3 >> 31);
       }
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++) qc[k] = qn[k];
     qm31 r = {{reduce63(s0), reduce63(s1)}, {reduce63(s2), reduce63(s3)}};
     r = wave_sum(r);
     if ((t & 63) == 0) wsum[g][t >> 6] = r;
