@@ -101,6 +101,21 @@ struct Hasher {
   }
 };
 
+// BLAKE2s-256 of a message of N <= 16 words (one block, so one compression with the
+// final flag and counter 4N); the indices are compile-time constants, so on the device the
+// message and state stay in registers (the streaming Hasher indexes its buffer at run time).
+template <int N>
+__host__ __device__ __forceinline__ void blake2s256_block(const uint32_t* words, uint32_t (&out)[8]) {
+  static_assert(N >= 1 && N <= 16, "one block");
+  uint32_t h[8] = {IV0 ^ 0x01010020u, IV1, IV2, IV3, IV4, IV5, IV6, IV7};
+  uint32_t m[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) m[i] = i < N ? words[i] : 0u;
+  compress(h, m, 4ull * N, true);
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = h[i];
+}
+
 // BLAKE2s-256 of a message given as little-endian 32-bit words (n_words * 4 bytes).
 __host__ __device__ inline void blake2s256_words(const uint32_t* words, uint32_t n_words, uint32_t* out) {
   Hasher hs;

@@ -260,7 +260,7 @@ constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
 constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead of 256; 1024 spills)
 template <int NT>
 __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
-                                                     Hash* const* outs, const LayerCols* inj) {
+                                                     Hash* const* outs, const LayerCols* inj, RootDraw rd) {
   extern __shared__ Hash s[];  // 2^w_log hashes (dynamic: shallow launches fit more blocks per CU)
   __shared__ uint32_t qmsg[NT / 4][16];  // quad mode: staging of injected column chunks
   const uint32_t nodes0 = 1u << w_log;
@@ -325,6 +325,16 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
       }
       __syncthreads();
     }
+  }
+  // the launch that wrote the root (layer 0, held in s[0] after the last barrier)
+  if (rd.ch && top + 1 == depth && threadIdx.x == 0) {
+    Channel ch = *rd.ch;
+    ch.mix8(s[0].s);
+    if (rd.point)
+      ch.draw_point(rd.dst[0], rd.dst[1]);
+    else
+      *rd.dst = ch.draw_qm31();
+    *rd.ch = ch;
   }
 }
 
@@ -407,13 +417,15 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   }
 }
 
-void MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready) const {
+bool MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready, const RootDraw* root_draw) const {
   size_t next = 0;
-  enqueue_from(&next, 0, layer_ready);
+  return enqueue_from(&next, 0, layer_ready, root_draw);
 }
 
-void MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready) const {
+bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready,
+                              const RootDraw* root_draw) const {
   hipStream_t st = ctx().cur;
+  bool drawn = false;
   for (; *next < steps.size() && lowest_layer(steps[*next]) >= min_layer; ++*next) {
     const Step& s = steps[*next];
     if (layer_ready) {  // the columns injected into the layers of this step must be written
@@ -442,12 +454,19 @@ void MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
       const uint32_t w_log =
           std::min(SUB_MAX_W_LOG, std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(log2_exact(SUB_NT), s.top)));
       const uint32_t blocks = 1u << (s.top - w_log);
+      // the root's launch (one block reaching layer 0) also runs the channel step
+      RootDraw rd{nullptr, nullptr, 0};
+      if (root_draw && blocks == 1 && s.top + 1 == s.depth) {
+        rd = *root_draw;
+        drawn = true;
+      }
       timed("merkle_subtree", [&] {
-        subtree_kernel<SUB_NT><<<blocks, SUB_NT, sizeof(Hash) << w_log, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts,
-                                                                  dlcs);
+        subtree_kernel<SUB_NT><<<blocks, SUB_NT, sizeof(Hash) << w_log, st>>>(outs[s.top + 1], s.top, s.depth, w_log,
+                                                                             douts, dlcs, rd);
       });
     }
   }
+  return drawn;
 }
 
 // Whole tree.  cols/logs are host arrays; layers[k] (host array of device buffers) gets 2^k hashes.

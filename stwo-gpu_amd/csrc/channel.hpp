@@ -33,14 +33,29 @@ struct Channel {
     hs.finish(digest);
     n_draws = 0;
   }
+  // mix(w, 8) with the message in registers (one block: digest || w)
+  __host__ __device__ void mix8(const uint32_t* w) {
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      m[i] = digest[i];
+      m[8 + i] = w[i];
+    }
+    b2s::blake2s256_block<16>(m, digest);
+    n_draws = 0;
+  }
   // Field-element vectors are mixed through their tree hash (blake2s.hpp): the prover
   // computes the root in parallel (prover.hip) and mixes it with mix(root, 8).
   __host__ __device__ void draw_words(uint32_t* out) {
     uint32_t m[10];
+#pragma unroll
     for (int i = 0; i < 8; i++) m[i] = digest[i];
     m[8] = n_draws++;
     m[9] = 0;
-    b2s::blake2s256_words(m, 10, out);
+    uint32_t h[8];
+    b2s::blake2s256_block<10>(m, h);
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = h[i];
   }
   __host__ __device__ qm31 draw_qm31() {
     for (;;) {
@@ -79,11 +94,12 @@ struct Channel {
 // proof-of-work check shared by the grinder and the host side
 __host__ __device__ inline uint32_t pow_trailing_zeros(const uint32_t* digest, uint64_t nonce) {
   uint32_t m[10];
+#pragma unroll
   for (int i = 0; i < 8; i++) m[i] = digest[i];
   m[8] = (uint32_t)nonce;
   m[9] = (uint32_t)(nonce >> 32);
   uint32_t h[8];
-  b2s::blake2s256_words(m, 10, h);
+  b2s::blake2s256_block<10>(m, h);
   uint32_t tz = 0;
   for (int k = 0; k < 2; k++) {
     if (h[k] == 0) {

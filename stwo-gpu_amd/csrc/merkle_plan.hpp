@@ -7,6 +7,7 @@
 
 #include <vector>
 
+#include "channel.hpp"
 #include "runtime.hpp"
 
 namespace sh {
@@ -14,6 +15,15 @@ namespace merkle {
 
 struct Hash {
   uint32_t s[8];
+};
+
+// After the root is written: mix it into a Fiat-Shamir channel (device memory) and draw a
+// QM31 into *dst, or a circle point into dst[0], dst[1] (`point`).  Run by the launch that
+// produces the root, it saves a single-lane kernel launch after every commitment.
+struct RootDraw {
+  Channel* ch;
+  qm31* dst;
+  uint32_t point;
 };
 
 // Columns injected into a layer: device array of column pointers.
@@ -52,12 +62,15 @@ struct MerklePlan {
   // per layer (or null) that marks its injected columns written, e.g. on another stream;
   // each step waits only for the layers it reads, so hashing starts while other
   // columns are still being produced.
-  void enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr) const;
+  // root_draw (optional): run by the launch that writes the root when that launch is a
+  // single-block subtree; returns whether it ran (else the caller mixes the root itself).
+  bool enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr, const RootDraw* root_draw = nullptr) const;
   // Enqueue steps from *next on while every layer they read is >= min_layer (the caller
   // has enqueued the producers of those layers); advances *next.  Interleaving these
   // calls with the producers' launches hands the hashing to the device as early as the
   // host submits its inputs.
-  void enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready) const;
+  bool enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready,
+                    const RootDraw* root_draw = nullptr) const;
   // lowest layer a step reads injected columns of
   static uint32_t lowest_layer(const Step& s) { return s.kind == Step::LAYER ? s.top : s.top + 1 - s.depth; }
 

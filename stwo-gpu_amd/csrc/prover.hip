@@ -96,11 +96,11 @@ __global__ void k_begin(JobBlock* jb, uint64_t seed, Params p, uint32_t n_line) 
   jb->ch.mix(st, 6);
 }
 __global__ void k_mix_root_draw(JobBlock* jb, const merkle::Hash* root, qm31* dst) {
-  jb->ch.mix(root->s, 8);
+  jb->ch.mix8(root->s);
   *dst = jb->ch.draw_qm31();
 }
 __global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root) {
-  jb->ch.mix(root->s, 8);
+  jb->ch.mix8(root->s);
   jb->ch.draw_point(jb->z[0], jb->z[1]);
 }
 // Tree root (blake2s.hpp) of a QM31 vector held as AoS (soa == false: v[i] = 4 words) or as
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, cons
   uint32_t root[8];
   block_tree_root(samples, Soa4c{}, false, n, root);
   if (threadIdx.x == 0) {
-    jb->ch.mix(root, 8);
+    jb->ch.mix8(root);
     jb->beta = jb->ch.draw_qm31();
     jb->fri_alpha[0] = jb->ch.draw_qm31();
   }
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, cons
 __global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
   uint32_t root[8];
   block_tree_root(nullptr, last, true, n, root);
-  if (threadIdx.x == 0) jb->ch.mix(root, 8);
+  if (threadIdx.x == 0) jb->ch.mix8(root);
 }
 __global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size) {
   const uint32_t w[2] = {(uint32_t)jb->nonce, (uint32_t)(jb->nonce >> 32)};
@@ -503,8 +503,11 @@ void Job::enqueue(uint64_t seed) {
   wide_fib_trace_async((uint32_t* const*)t_coeffs, C, ln, &jb->seed);
   fft::lde_same_size(t_coeffs, (uint32_t* const*)t_coeffs, (uint32_t* const*)t_lde, C, ln, 1, tw, itw, tree_len,
                      fft::lde_variant_for(ln, 1, C));
-  tree_t->enqueue();
-  timed("channel_trace_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
+  {
+    const merkle::RootDraw rd{&jb->ch, &jb->alpha, 0};
+    if (!tree_t->enqueue(nullptr, &rd))
+      timed("channel_trace_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
+  }
 
   qm31_powers_async(&jb->alpha, C - 2, powers);
   vanishing_inverse_async(dinv, ln + 1, ln);
@@ -512,8 +515,11 @@ void Job::enqueue(uint64_t seed) {
   wide_fib_quotients_async(q4, t_lde, powers, dinv, (uint32_t)(2 * n), C);
   fft::lde_same_size(t_comp, (uint32_t* const*)t_comp, (uint32_t* const*)t_comp_lde, 4, ln + 1, 1, tw, itw, tree_len,
                      fft::lde_variant_for(ln + 1, 1, 4));
-  tree_q->enqueue();
-  timed("channel_oods_point", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
+  {
+    const merkle::RootDraw rd{&jb->ch, jb->z, 1};
+    if (!tree_q->enqueue(nullptr, &rd))
+      timed("channel_oods_point", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
+  }
 
   eval_at_point_columns_async(t_coeffs, C, ln, jb->z, samples, eap_scratch);
   eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch);
@@ -536,8 +542,10 @@ void Job::enqueue(uint64_t seed) {
                               false);
   for (uint32_t j = 1; j <= L.n_line; j++) {
     const uint32_t m = ln + 2 - j;
-    tree_l[j - 1]->enqueue();
-    timed("channel_fri_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
+    const merkle::RootDraw rd{&jb->ch, &jb->fri_alpha[j], 0};
+    if (!tree_l[j - 1]->enqueue(nullptr, &rd))
+      timed("channel_fri_root",
+            [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
     auto src = coords(line[j - 1], m), dst = coords(line[j], m - 1);
     fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j]);
     if (j == 1)
