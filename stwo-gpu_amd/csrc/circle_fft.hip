@@ -55,6 +55,9 @@ struct PassArgs {
   uint32_t lob_count_log;  // log2 of the number of low blocks per column (a - t)
 };
 
+// accelerator complex dies (XCDs) of gfx950, each with its own L2
+constexpr uint32_t N_XCD = 8;
+
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (I < N) {
@@ -98,10 +101,16 @@ struct Pass {
       tile = blockIdx.x / A.n_groups;
     } else {
       // bands of `band` columns: the blocks in flight share each tile's twiddles across the
-      // band and touch few columns (pages) at a time
-      const uint32_t tiles = gridDim.x / A.n_groups;
+      // band and touch few columns (pages) at a time.  Blocks go round-robin to the 8 XCDs
+      // (block i to XCD i % 8), each with its own L2, so each XCD takes a contiguous range of
+      // this order: the band members of one tile then run on one XCD and read its twiddles
+      // from that XCD's L2 once.
+      const uint32_t G = gridDim.x, q = G / N_XCD, rem = G % N_XCD;
+      const uint32_t x = blockIdx.x % N_XCD;
+      const uint32_t v = x * q + min(x, rem) + blockIdx.x / N_XCD;
+      const uint32_t tiles = G / A.n_groups;
       const uint32_t per_band = A.band * tiles;
-      const uint32_t bi = blockIdx.x / per_band, r = blockIdx.x - bi * per_band;
+      const uint32_t bi = v / per_band, r = v - bi * per_band;
       const uint32_t w = min(A.band, A.n_groups - bi * A.band);
       col_group = bi * A.band + r % w;
       tile = r / w;
