@@ -200,6 +200,8 @@ int stwo_hip_fill_uniform_m31(uint32_t *dst, uint64_t n, uint64_t seed, uint64_t
  * stwo_hip_release_twiddles); both trees have 2^log_size entries. */
 int stwo_hip_twiddles(uint32_t initial_index, uint32_t step_size, uint32_t log_size,
                       const uint32_t **twiddles, const uint32_t **itwiddles);
+/* Frees every cached twiddle tree and the cached quotient domain tables (one small table
+ * pair per circle domain that accumulate_quotients has run over). */
 int stwo_hip_release_twiddles(void);
 
 /* Asynchronous batched circle FFTs on the library stream.  Columns may have different log

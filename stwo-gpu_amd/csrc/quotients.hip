@@ -15,6 +15,7 @@
 // a row costs one QM31 x M31 multiply-accumulate per sample, done on 64-bit
 // lazy sums (v_mad_u64_u32) and reduced once per batch.  Sample tables are
 // uniform across the wavefront and are read with scalar loads.
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -277,6 +278,25 @@ static quot::DomainTables domain_tables_async(uint32_t hi, uint32_t hs, uint32_t
   return quot::DomainTables{lo, up, k, hl};
 }
 
+// The domain tables of CircleDomain(half coset (hi, hs), log log_size), computed on first
+// use and kept in the library context (until stwo_hip_release_twiddles), so repeated
+// accumulations over one domain launch no table kernel.
+static quot::DomainTables cached_domain_tables(uint32_t hi, uint32_t hs, uint32_t log_size) {
+  const uint32_t hl = log_size - 1, k = quot::dt_k(hl);
+  const TwiddleKey key{hi, hs, log_size};
+  auto& cache = ctx().domain_tables;
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    void* dev = ctx().pool.alloc(quot::dt_bytes(hl));
+    domain_tables_async(hi, hs, log_size, dev);
+    // the cached tables are read by work on every stream: finish them before sharing
+    hip_check(hipStreamSynchronize(ctx().cur), "hipStreamSynchronize");
+    it = cache.emplace(key, dev).first;
+  }
+  pt* lo = (pt*)it->second;
+  return quot::DomainTables{lo, lo + (1u << k), k, hl};
+}
+
 // accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
 // the point (x, y), the sample values and the random coefficient read from device memory.
 void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
@@ -326,13 +346,18 @@ void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t*
   while (slots.empty() || slots.size() % quot::QG) slots.push_back(pad);
   const uint32_t n_loop = (uint32_t)slots.size();
   for (int i = 0; i < quot::QG; i++) slots.push_back(pad);  // read by the last group's prefetch
-  DevBuf dsl = upload(slots.data(), slots.size() * sizeof(quot::ColSlot<NB>));
-  DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
-  DevBuf dt(quot::dt_bytes(log_size - 1));
-  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, dt.p);
+  // one upload: slots, then the batches at a 256-byte boundary
+  const size_t slots_bytes = (slots.size() * sizeof(quot::ColSlot<NB>) + 255) & ~(size_t)255;
+  std::vector<unsigned char> host(slots_bytes + bs.size() * sizeof(quot::Batch));
+  memcpy(host.data(), slots.data(), slots.size() * sizeof(quot::ColSlot<NB>));
+  memcpy(host.data() + slots_bytes, bs.data(), bs.size() * sizeof(quot::Batch));
+  DevBuf d = upload(host.data(), host.size());
+  const quot::ColSlot<NB>* dsl = d.as<quot::ColSlot<NB>>();
+  const quot::Batch* dbs = (const quot::Batch*)((const unsigned char*)d.p + slots_bytes);
+  const quot::DomainTables T = cached_domain_tables(hi, hs, log_size);
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<NB><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        T, log_size, dsl.as<quot::ColSlot<NB>>(), n_loop, dbs.as<quot::Batch>(), out[0], out[1], out[2], out[3]);
+        T, log_size, dsl, n_loop, dbs, out[0], out[1], out[2], out[3]);
   });
 }
 void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,
@@ -348,9 +373,10 @@ void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* co
 }  // namespace
 
 // points: n_batches x StwoCirclePointQM31 (host); indexes/values flattened in batch order (host).
-// d_cols: device table of the columns; h_cols: the same pointers in host memory.
+// h_cols: the n_cols column pointers in host memory; d_cols: the same table in device memory,
+// or null (uploaded here when the batch-major kernel needs it).
 void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, const uint32_t* const* d_cols,
-                                const uint32_t* const* h_cols,
+                                const uint32_t* const* h_cols, uint32_t n_cols,
                                 qm31 rc, uint32_t n_batches, const StwoCirclePointQM31* points,
                                 const uint32_t* sizes, const uint32_t* col_idx, const StwoQM31* values,
                                 uint32_t* const out[4]) {
@@ -393,14 +419,22 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
   }
   if (ss.empty()) ss.push_back(quot::Sample{0, 0, qzero()});
   if (bs.empty()) bs.push_back(quot::Batch{});
-  DevBuf dss = upload(ss.data(), ss.size() * sizeof(quot::Sample));
-  DevBuf dbs = upload(bs.data(), bs.size() * sizeof(quot::Batch));
-  DevBuf dt(quot::dt_bytes(log_size - 1));
-  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, dt.p);
+  // one upload: samples, batches and (when the caller has no device table) the columns
+  const size_t ss_bytes = (ss.size() * sizeof(quot::Sample) + 255) & ~(size_t)255;
+  const size_t bs_bytes = (bs.size() * sizeof(quot::Batch) + 255) & ~(size_t)255;
+  const size_t cols_bytes = d_cols ? 0 : (size_t)n_cols * sizeof(void*);
+  std::vector<unsigned char> host(ss_bytes + bs_bytes + cols_bytes);
+  memcpy(host.data(), ss.data(), ss.size() * sizeof(quot::Sample));
+  memcpy(host.data() + ss_bytes, bs.data(), bs.size() * sizeof(quot::Batch));
+  if (cols_bytes) memcpy(host.data() + ss_bytes + bs_bytes, h_cols, cols_bytes);
+  DevBuf d = upload(host.data(), host.size());
+  const unsigned char* base = (const unsigned char*)d.p;
+  if (!d_cols) d_cols = (const uint32_t* const*)(base + ss_bytes + bs_bytes);
+  const quot::DomainTables T = cached_domain_tables(hi, hs, log_size);
   timed("accumulate_quotients", [&] {
     quot::accumulate_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().cur>>>(
-        T, log_size, d_cols, dss.as<quot::Sample>(), dbs.as<quot::Batch>(), n_batches, out[0], out[1], out[2],
-        out[3]);
+        T, log_size, d_cols, (const quot::Sample*)base, (const quot::Batch*)(base + ss_bytes), n_batches, out[0],
+        out[1], out[2], out[3]);
   });
 }
 
@@ -419,10 +453,9 @@ extern "C" int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_i
     for (uint32_t b = 0; b < n_batches; b++) total += batch_sizes[b];
     for (uint64_t i = 0; i < total; i++)
       require(column_indexes[i] < n_columns, "accumulate_quotients: column index out of range");
-    DevBuf table = upload(columns, (size_t)n_columns * sizeof(void*));
     uint32_t* o[4] = {out[0], out[1], out[2], out[3]};
-    accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
-                               table.as<const uint32_t*>(), columns, qm31{{random_coeff.a, random_coeff.b},
+    accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size, nullptr, columns,
+                               n_columns, qm31{{random_coeff.a, random_coeff.b},
                                                                  {random_coeff.c, random_coeff.d}},
                                n_batches, points, batch_sizes, column_indexes, values, o);
   });
@@ -453,7 +486,8 @@ extern "C" void accumulate_quotients(uint32_t half_coset_initial_index, uint32_t
       sync();
     }
     accumulate_quotients_async(half_coset_initial_index, half_coset_step_size, domain_size,
-                               (const uint32_t* const*)columns, host_cols.data(), qm31{{random_coeff.a, random_coeff.b},
+                               (const uint32_t* const*)columns, host_cols.data(), (uint32_t)number_of_columns,
+                               qm31{{random_coeff.a, random_coeff.b},
                                                                      {random_coeff.c, random_coeff.d}},
                                sample_size, (const StwoCirclePointQM31*)sample_points, sample_column_and_values_sizes,
                                sample_column_indexes, sample_column_values, out);

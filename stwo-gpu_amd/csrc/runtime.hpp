@@ -94,6 +94,9 @@ struct Context {
   hipEvent_t fork = nullptr, join[N_AUX_STREAMS] = {};
   Pool pool;
   std::map<TwiddleKey, TwiddleEntry> twiddles;
+  // domain point tables of the circle domains that quotients were accumulated over,
+  // keyed like the twiddle trees (half coset initial index, step, domain log size)
+  std::map<TwiddleKey, void*> domain_tables;
   bool kernel_timing = false;
   std::vector<TimedLaunch> pending;
   std::map<std::string, std::pair<uint64_t, double>> stats;

@@ -101,5 +101,7 @@ extern "C" int stwo_hip_release_twiddles(void) {
       ctx().pool.free(kv.second.itw);
     }
     ctx().twiddles.clear();
+    for (auto& kv : ctx().domain_tables) ctx().pool.free(kv.second);
+    ctx().domain_tables.clear();
   });
 }

@@ -472,14 +472,17 @@ def test_async_fri_chain_matches_oracle():
         assert (layers[i].to_numpy() == o).all()
 
 
-def test_async_quotients_match_oracle():
-    rng = np.random.default_rng(11)
-    m, n_cols = 13, 12
+@pytest.mark.parametrize("m,sizes", [(13, [4, 7, 1]), (12, [4, 7, 1, 2, 3, 5]), (13, [2, 2, 2, 2, 2, 2])])
+def test_async_quotients_match_oracle(m, sizes):
+    # up to 4 batches: column-major kernel; more: batch-major kernel with the column table
+    # uploaded by the call.  The third case reuses the first one's cached domain tables.
+    rng = np.random.default_rng(11 + len(sizes))
+    n_cols = 12
     root = S.canonic_half_coset(m)
     cols = [rand_m31(1 << m, 900 + i) for i in range(n_cols)]
-    pts = [S.random_circle_point_qm31(rng) for _ in range(3)]
+    pts = [S.random_circle_point_qm31(rng) for _ in range(len(sizes))]
     batches = [(pts[b], [(int(c), S.random_qm31(rng)) for c in sorted(rng.choice(n_cols, size=k, replace=False))])
-               for b, k in enumerate([4, 7, 1])]
+               for b, k in enumerate(sizes)]
     rc = S.random_qm31(rng)
     out = S.SecureColumn.uninitialized(1 << m)
     S.accumulate_quotients_async(root, [S.BaseFieldVec.from_numpy(c) for c in cols], rc, S.QuotientSamples(batches),
