@@ -9,9 +9,11 @@
 // (blake2s.cu:56-95).
 //
 // MI355X design: one lane per node; a wavefront reads 64 consecutive rows of a
-// column (256-byte coalesced loads); the whole tree below the leaf layer is
-// built by a subtree kernel that produces up to 11 layers per launch in LDS,
-// injecting the columns of any layer (mixed-degree commitments) as it goes.
+// column (256-byte coalesced loads); the tree below the wide layers is built by
+// subtree launches over runs of light layers, up to 11 layers per launch in LDS; a
+// run topped by at most 2^12 nodes takes one launch: blocks build 128-node subtrees
+// and the last block to finish builds the levels above their roots.  Columns of any
+// layer (mixed-degree commitments) are injected as the layers are built.
 #include <algorithm>
 #include <cstring>
 
@@ -249,28 +251,24 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
   out[i].s[4 + q] = h_hi;
 }
 
-// Subtree kernel: layers top, top-1, ..., top-depth+1 (each halving).  Block b reads
-// 2^(w_log+1) children hashes of layer top+1 (prev), produces 2^w_log nodes of layer
-// `top` and halves them in LDS down to 2^(w_log-depth+1) nodes of layer top-depth+1,
-// i.e. it builds 2^(w_log-depth+1) adjacent subtrees side by side (w_log >= depth-1;
-// shallow runs are packed so the first layer still spans the whole block).  outs[k]
-// and inj[k] are indexed by layer k.
-constexpr int SUB_MAX_DEPTH = 11;  // 1024 nodes in the first produced layer
-constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
-constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead of 256; 1024 spills)
+// Subtree levels: layers top, top-1, ..., top-depth+1 (each halving), built by one block.
+// The block reads 2^(w+1) children hashes of layer top+1 from `prev` (global memory)
+// starting at child 2 base0, produces the nodes0 = 2^w nodes of layer `top` starting at
+// base0 and halves them in LDS down to 2^(w-depth+1) nodes of layer top-depth+1, i.e. it
+// builds 2^(w-depth+1) adjacent subtrees side by side (w >= depth-1).  Wide levels take one
+// lane per node; levels of at most NT/4 nodes one lane quad per node (the four G
+// functions of each half-round side by side).  outs[k] and inj[k] are indexed by layer k;
+// s holds nodes0 hashes.
 template <int NT>
-__global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
-                                                     Hash* const* outs, const LayerCols* inj, RootDraw rd) {
-  extern __shared__ Hash s[];  // 2^w_log hashes (dynamic: shallow launches fit more blocks per CU)
-  __shared__ uint32_t qmsg[NT / 4][16];  // quad mode: staging of injected column chunks
-  const uint32_t nodes0 = 1u << w_log;
-  const uint32_t base0 = blockIdx.x << w_log;
+__device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uint32_t depth, uint32_t nodes0,
+                                             uint32_t base0, Hash* const* outs, const LayerCols* inj, Hash* s,
+                                             uint32_t (*qmsg)[16]) {
   for (uint32_t d = 0; d < depth; d++) {
     const uint32_t layer = top - d;
     const uint32_t cnt = nodes0 >> d;
     const uint32_t base = base0 >> d;
     const LayerCols lc = inj[layer];
-    if (d > 0 && cnt <= blockDim.x / 4) {
+    if (d > 0 && cnt <= NT / 4) {
       // narrow level: one quad per node, children read from s in place
       const uint32_t i = threadIdx.x >> 2, q = threadIdx.x & 3;
       uint32_t h_lo = 0, h_hi = 0;
@@ -300,7 +298,7 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
     }
     // one node per lane per round; a round's results are written back after a barrier, and
     // later rounds read children at indices above everything written so far
-    for (uint32_t r0 = 0; r0 < cnt; r0 += blockDim.x) {
+    for (uint32_t r0 = 0; r0 < cnt; r0 += NT) {
       const uint32_t i = r0 + threadIdx.x;
       uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       if (i < cnt) {
@@ -326,6 +324,46 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
       __syncthreads();
     }
   }
+}
+
+// Subtree launch over layers top .. top-depth+1.  Block b builds the levels of its
+// 2^w_log nodes of layer `top` (build_levels).  With `arrive` (a counter that is zero
+// at launch), each block builds w_log + 1 levels, down to one root per block, and the
+// last block to finish builds the remaining depth - w_log - 1 levels from all block
+// roots, so a whole tree top needs one launch whose wide levels still spread over the
+// chip.  No block waits for another: each block publishes its nodes (device-scope fence)
+// and counts itself in; the block that counts last continues, the others exit.  The last
+// block resets the counter for the next launch.
+constexpr uint32_t SUB_BLOCK_LOG = 7;  // nodes per block at the top layer (one lane quad each below)
+// Runs topped by at most 2^12 nodes (at most 32 blocks) finish in their last block.  The
+// device-scope fence of every block writes its XCD's L2 back, which costs more than a
+// second launch once there are hundreds of blocks: wider runs are cut after
+// SUB_MAX_DEPTH layers into blocks of 2^9..2^10 nodes, and the layers above them form
+// the next run.
+constexpr uint32_t TAIL_MAX_TOP = 12;
+constexpr uint32_t SUB_MAX_DEPTH = 11;
+constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
+constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead of 256; 1024 spills)
+template <int NT>
+__global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
+                                                     Hash* const* outs, const LayerCols* inj, RootDraw rd,
+                                                     uint32_t* arrive) {
+  extern __shared__ Hash s[];  // max(2^w_log, block count / 2) hashes
+  __shared__ uint32_t qmsg[NT / 4][16];  // quad mode: staging of injected column chunks
+  __shared__ uint32_t is_last;
+  const uint32_t bd = arrive ? w_log + 1 : depth;  // levels built by every block
+  build_levels<NT>(prev, top, bd, 1u << w_log, blockIdx.x << w_log, outs, inj, s, qmsg);
+  if (arrive) {
+    __threadfence();  // this block's nodes are visible device-wide before it counts in
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(arrive, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!is_last) return;  // uniform over the block
+    __threadfence();       // every other block's nodes are visible to the loads below
+    if (threadIdx.x == 0) *arrive = 0u;
+    const uint32_t t_top = top - bd;  // the block roots are layer t_top + 1
+    build_levels<NT>(outs[t_top + 1], t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg);
+  }
   // the launch that wrote the root (layer 0, held in s[0] after the last barrier)
   if (rd.ch && top + 1 == depth && threadIdx.x == 0) {
     Channel ch = *rd.ch;
@@ -350,7 +388,8 @@ constexpr int32_t WIDE_LAYER = 16;
 size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log) {
   const size_t cols = ((size_t)std::max<uint32_t>(n_cols, 1) * sizeof(void*) + 15) & ~(size_t)15;
   const size_t lcs = ((size_t)(max_log + 1) * sizeof(LayerCols) + 15) & ~(size_t)15;
-  return cols + lcs + (size_t)(max_log + 1) * sizeof(Hash*);
+  const size_t outs = ((size_t)(max_log + 1) * sizeof(Hash*) + 15) & ~(size_t)15;
+  return cols + lcs + outs + (size_t)(max_log + 1) * sizeof(uint32_t);
 }
 
 MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
@@ -395,8 +434,13 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   memcpy(host, flat.data(), flat.size() * sizeof(void*));
   memcpy(host + o_lcs, lcs.data(), lcs.size() * sizeof(LayerCols));
   memcpy(host + o_outs, outs.data(), outs.size() * sizeof(Hash*));
+  // arrival counters of the subtree launches that finish in their last block, by top layer;
+  // zero at upload, and each launch leaves its counter at zero
+  const size_t o_arrive = o_outs + (((size_t)(max_log + 1) * sizeof(Hash*) + 15) & ~(size_t)15);
+  memset(host + o_arrive, 0, (size_t)(max_log + 1) * sizeof(uint32_t));
   dlcs = (const LayerCols*)(dev + o_lcs);
   douts = (Hash* const*)(dev + o_outs);
+  darrive = (uint32_t*)(dev + o_arrive);
   // launch schedule, top-down: wide layers (more than 2^16 nodes, enough lanes to fill
   // the chip) and injection-heavy layers get one launch each, runs of the remaining
   // layers are built by subtree launches
@@ -408,9 +452,9 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
       top--;
       continue;
     }
+    const uint32_t max_depth = (uint32_t)top <= TAIL_MAX_TOP ? (uint32_t)top + 1 : SUB_MAX_DEPTH;
     uint32_t depth = 1;
-    while (depth < (uint32_t)SUB_MAX_DEPTH && (int32_t)depth <= top &&
-           !((lcs[top - depth].n_cols + 15) / 16 >= HEAVY_CHUNKS && top - (int32_t)depth >= 8))
+    while (depth < max_depth && !((lcs[top - depth].n_cols + 15) / 16 >= HEAVY_CHUNKS && top - (int32_t)depth >= 8))
       depth++;
     steps.push_back({Step::SUBTREE, (uint32_t)top, depth});
     top -= (int32_t)depth;
@@ -450,19 +494,26 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
                                                                      outs[k]);
       });
     } else {
-      // at least SUB_NT nodes of layer top per block (or the whole layer when it is smaller)
+      // narrow runs: 2^SUB_BLOCK_LOG nodes of layer top per block (the whole layer when it
+      // is smaller), and runs deeper than one block's subtree finish in the last block;
+      // wide runs: at least SUB_NT nodes per block
+      const bool narrow = s.top <= TAIL_MAX_TOP;
       const uint32_t w_log =
-          std::min(SUB_MAX_W_LOG, std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(log2_exact(SUB_NT), s.top)));
+          narrow ? std::min<uint32_t>(SUB_BLOCK_LOG, s.top)
+                 : std::min(SUB_MAX_W_LOG, std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(log2_exact(SUB_NT), s.top)));
       const uint32_t blocks = 1u << (s.top - w_log);
-      // the root's launch (one block reaching layer 0) also runs the channel step
+      const bool tail = s.depth > w_log + 1;
+      require(!tail || (narrow && blocks >= 2), "merkle: bad subtree step");
+      const size_t lds = sizeof(Hash) * std::max<size_t>(1u << w_log, tail ? blocks / 2 : 0);
+      // the launch that writes the root also runs the channel step
       RootDraw rd{nullptr, nullptr, 0};
-      if (root_draw && blocks == 1 && s.top + 1 == s.depth) {
+      if (root_draw && s.top + 1 == s.depth) {
         rd = *root_draw;
         drawn = true;
       }
       timed("merkle_subtree", [&] {
-        subtree_kernel<SUB_NT><<<blocks, SUB_NT, sizeof(Hash) << w_log, st>>>(outs[s.top + 1], s.top, s.depth, w_log,
-                                                                             douts, dlcs, rd);
+        subtree_kernel<SUB_NT><<<blocks, SUB_NT, lds, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs, rd,
+                                                            tail ? darrive + s.top : nullptr);
       });
     }
   }

@@ -42,6 +42,7 @@ struct MerklePlan {
   DevBuf own;                     // the device tables, unless they live in caller memory
   const LayerCols* dlcs = nullptr;  // device: per-layer column tables
   Hash* const* douts = nullptr;     // device: layer output pointers
+  uint32_t* darrive = nullptr;      // device: per-layer arrival counters of subtree launches
   std::vector<LayerCols> lcs;       // host copies (device column-table pointers)
   std::vector<Hash*> outs;
   std::vector<Step> steps;
@@ -62,8 +63,8 @@ struct MerklePlan {
   // per layer (or null) that marks its injected columns written, e.g. on another stream;
   // each step waits only for the layers it reads, so hashing starts while other
   // columns are still being produced.
-  // root_draw (optional): run by the launch that writes the root when that launch is a
-  // single-block subtree; returns whether it ran (else the caller mixes the root itself).
+  // root_draw (optional): run by the subtree launch that writes the root; returns whether
+  // it ran (false when the root is written by a layer launch: the caller mixes it itself).
   bool enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr, const RootDraw* root_draw = nullptr) const;
   // Enqueue steps from *next on while every layer they read is >= min_layer (the caller
   // has enqueued the producers of those layers); advances *next.  Interleaving these

@@ -493,12 +493,19 @@ def test_async_quotients_match_oracle(m, sizes):
 
 
 # ------------------------------------------------------------------ batched commit (poly.rs:57-168 + blake2s.rs:19-97)
-@pytest.mark.parametrize("use_graph,autotune", [(False, False), (True, False), (True, True)])
-def test_commit_plan_mixed_sizes(use_graph, autotune):
+MIXED_LOGS = [15, 3, 10, 15, 1, 10, 10, 14, 6, 12, 15, 2]
+# a tree of 2^13 leaves: the subtree launch over layers 12..0 finishes in its last block,
+# with columns injected below and above the block roots (layer 5)
+NARROW_LOGS = [12, 3, 1, 7, 12, 2, 9, 4]
+
+
+@pytest.mark.parametrize("use_graph,autotune,logs", [(False, False, MIXED_LOGS), (True, False, MIXED_LOGS),
+                                                     (True, True, MIXED_LOGS), (False, False, NARROW_LOGS),
+                                                     (True, False, NARROW_LOGS)])
+def test_commit_plan_mixed_sizes(use_graph, autotune, logs):
     """LDE + Merkle over columns of mixed sizes in one plan; every layer equals the
     oracle's tree over the oracle's LDE, and graph replays reproduce it."""
     blow = 1
-    logs = [15, 3, 10, 15, 1, 10, 10, 14, 6, 12, 15, 2]
     m = max(logs) + blow
     root = S.canonic_half_coset(m)
     tw, itw = S.cached_twiddles(root)
