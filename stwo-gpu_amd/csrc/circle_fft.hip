@@ -379,22 +379,27 @@ constexpr int BS_LDE = 14;
 // The fused middle pass saves one HBM read of the coefficients, but it holds the
 // coefficients and the forward tile in registers.
 constexpr int NT_S = 1024;  // threads per 2^14-element tile: 16 values per thread
-// Tile bits of the fused pass for K layers: 2^13-element tiles (8 values per thread, 32-word
-// runs) while K <= 8, so the kept coefficients and the forward copy fit in 64 VGPRs and two
-// blocks share a CU (256 columns of log 20: 1.39 ms against 1.63 ms for the separate
-// strided passes, the default for log 14..20); 2^14 for K = 9 (98 VGPRs, one block per
-// CU: a variant for the autotuner only).
+// Tile bits and threads of the fused pass for K layers: while K <= 8, 2^13-element tiles
+// (32-word runs) on 512 threads, 16 values per thread (two radix-8 groups per stage), so
+// the kept coefficients and the forward copy fit in 90 VGPRs and two blocks share a CU
+// (256 columns of log 20: 1.33 ms, against 1.36 ms on 1024 threads with 8 values each
+// and 1.63 ms for the separate strided passes; the default for log 14..20); 2^14 on 1024
+// threads for K = 9 (98 VGPRs, one block per CU: a variant for the autotuner only).
 template <int K>
 constexpr int lde_tile_bits() {
   return K <= 8 ? 13 : BS_LDE;
 }
 template <int K>
-__global__ __launch_bounds__(NT_S) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 8 : 4)))
+constexpr int lde_threads() {
+  return K <= 8 ? 512 : NT_S;
+}
+template <int K>
+__global__ __launch_bounds__(lde_threads<K>()) __attribute__((amdgpu_waves_per_eu(K <= 8 ? 5 : 4)))
 void fft_lde_strided_kernel(PassArgs Ai, PassArgs Af, uint32_t n_blocks) {
   constexpr int BL = lde_tile_bits<K>();
   __shared__ uint32_t s[1 << BL];
-  using PI = Pass<true, BL, NT_S, K, STRIDED>;
-  using PF = Pass<false, BL, NT_S, K, STRIDED, false>;
+  using PI = Pass<true, BL, lde_threads<K>(), K, STRIDED>;
+  using PF = Pass<false, BL, lde_threads<K>(), K, STRIDED, false>;
   constexpr int E = PI::E, NCH = PI::NCH;
   constexpr int LO_TOP = 3 * (NCH - 1), W_TOP = K - LO_TOP;
   // each stage's twiddles are loaded before the exchange that precedes it (as in Pass::run)
@@ -661,7 +666,7 @@ template <int K>
 static void launch_lde_strided(uint32_t blocks, const PassArgs& Ai, const PassArgs& Af, uint32_t n_blocks, uint32_t k) {
   if constexpr (K >= 2) {
     if (k == K) {
-      fft_lde_strided_kernel<K><<<blocks, NT_S, 0, ctx().cur>>>(Ai, Af, n_blocks);
+      fft_lde_strided_kernel<K><<<blocks, lde_threads<K>(), 0, ctx().cur>>>(Ai, Af, n_blocks);
       return;
     }
     launch_lde_strided<K - 1>(blocks, Ai, Af, n_blocks, k);
