@@ -270,7 +270,9 @@ struct Pass {
 
   // LDS exchanges use the plain position as address.  Chunks 0 and 1 of the contiguous
   // passes are 8-way bank conflicted; an XOR swizzle that removes every conflict costs
-  // ~150 VALU instructions per thread and measured slower (these passes are VALU-bound).
+  // ~150 VALU instructions per thread and measured slower (these passes are VALU-bound),
+  // and 8 words of padding after every 64 (~20 instructions per thread, conflict free)
+  // measured no faster: the conflicted exchanges overlap other waves' butterflies.
   template <int LO, int W>
   __device__ __forceinline__ void lds_write(const uint32_t (&v)[E]) const {
     constexpr int R = 1 << W;
