@@ -10,9 +10,9 @@
 //
 // MI355X design: one lane per node; a wavefront reads 64 consecutive rows of a
 // column (256-byte coalesced loads); the tree below the wide layers is built by
-// subtree launches over runs of light layers, up to 11 layers per launch in LDS; a
-// run topped by at most 2^12 nodes takes one launch: blocks build 128-node subtrees
-// and the last block to finish builds the levels above their roots.  Columns of any
+// subtree launches over runs of light layers in LDS; a run topped by at most 2^12
+// nodes takes one launch: blocks build 128-node subtrees and the last block to finish
+// builds the levels above their roots; wider runs take one launch per 8-9 layers.  Columns of any
 // layer (mixed-degree commitments) are injected as the layers are built.
 #include <algorithm>
 #include <cstring>
@@ -337,12 +337,13 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
 constexpr uint32_t SUB_BLOCK_LOG = 7;  // nodes per block at the top layer (one lane quad each below)
 // Runs topped by at most 2^12 nodes (at most 32 blocks) finish in their last block.  The
 // device-scope fence of every block writes its XCD's L2 back, which costs more than a
-// second launch once there are hundreds of blocks: wider runs are cut after
-// SUB_MAX_DEPTH layers into blocks of 2^9..2^10 nodes, and the layers above them form
-// the next run.
+// second launch once there are hundreds of blocks: a wider run is cut into at most
+// 2^SUB_WIDE_BLOCKS_LOG blocks (one per CU; 149 VGPRs keep one 512-thread block per CU)
+// of 2^7..2^8 nodes, each reduced to its root, and the layers above the roots form the
+// next run.
 constexpr uint32_t TAIL_MAX_TOP = 12;
-constexpr uint32_t SUB_MAX_DEPTH = 11;
-constexpr uint32_t SUB_MAX_W_LOG = SUB_MAX_DEPTH - 1;
+constexpr uint32_t SUB_WIDE_BLOCKS_LOG = 8;
+static uint32_t wide_w_log(uint32_t top) { return std::max<uint32_t>(top - SUB_WIDE_BLOCKS_LOG, SUB_BLOCK_LOG); }
 constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead of 256; 1024 spills)
 template <int NT>
 __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
@@ -452,7 +453,7 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
       top--;
       continue;
     }
-    const uint32_t max_depth = (uint32_t)top <= TAIL_MAX_TOP ? (uint32_t)top + 1 : SUB_MAX_DEPTH;
+    const uint32_t max_depth = (uint32_t)top <= TAIL_MAX_TOP ? (uint32_t)top + 1 : wide_w_log((uint32_t)top) + 1;
     uint32_t depth = 1;
     while (depth < max_depth && !((lcs[top - depth].n_cols + 15) / 16 >= HEAVY_CHUNKS && top - (int32_t)depth >= 8))
       depth++;
@@ -496,11 +497,9 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
     } else {
       // narrow runs: 2^SUB_BLOCK_LOG nodes of layer top per block (the whole layer when it
       // is smaller), and runs deeper than one block's subtree finish in the last block;
-      // wide runs: at least SUB_NT nodes per block
+      // wide runs: one subtree per block, at most 2^SUB_WIDE_BLOCKS_LOG blocks
       const bool narrow = s.top <= TAIL_MAX_TOP;
-      const uint32_t w_log =
-          narrow ? std::min<uint32_t>(SUB_BLOCK_LOG, s.top)
-                 : std::min(SUB_MAX_W_LOG, std::max<uint32_t>(s.depth - 1, std::min<uint32_t>(log2_exact(SUB_NT), s.top)));
+      const uint32_t w_log = narrow ? std::min<uint32_t>(SUB_BLOCK_LOG, s.top) : wide_w_log(s.top);
       const uint32_t blocks = 1u << (s.top - w_log);
       const bool tail = s.depth > w_log + 1;
       require(!tail || (narrow && blocks >= 2), "merkle: bad subtree step");
