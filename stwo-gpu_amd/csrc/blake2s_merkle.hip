@@ -253,7 +253,8 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
 
 // Subtree levels: layers top, top-1, ..., top-depth+1 (each halving), built by one block.
 // The block reads 2^(w+1) children hashes of layer top+1 from `prev` (global memory)
-// starting at child 2 base0, produces the nodes0 = 2^w nodes of layer `top` starting at
+// starting at child 2 base0 (none when `prev` is null: layer `top` is the leaf layer),
+// produces the nodes0 = 2^w nodes of layer `top` starting at
 // base0 and halves them in LDS down to 2^(w-depth+1) nodes of layer top-depth+1, i.e. it
 // builds 2^(w-depth+1) adjacent subtrees side by side (w >= depth-1).  Wide levels take one
 // lane per node; levels of at most NT/4 nodes one lane quad per node (the four G
@@ -302,17 +303,19 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
       const uint32_t i = r0 + threadIdx.x;
       uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       if (i < cnt) {
-        uint32_t m[16];
-        if (d == 0) {
-          load_children(prev, base + i, m);
-        } else {
+        if (d > 0 || prev) {  // leaves (no prev) have no children to compress
+          uint32_t m[16];
+          if (d == 0) {
+            load_children(prev, base + i, m);
+          } else {
 #pragma unroll
-          for (int w = 0; w < 8; w++) {
-            m[w] = s[2 * i].s[w];
-            m[8 + w] = s[2 * i + 1].s[w];
+            for (int w = 0; w < 8; w++) {
+              m[w] = s[2 * i].s[w];
+              m[8 + w] = s[2 * i + 1].s[w];
+            }
           }
+          compress(h, m);
         }
-        compress(h, m);
         absorb_columns_simple(h, lc.cols, lc.n_cols, base + i);
         store_hash(outs[layer], base + i, h);
       }
@@ -445,8 +448,11 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   // launch schedule, top-down: wide layers (more than 2^16 nodes, enough lanes to fill
   // the chip) and injection-heavy layers get one launch each, runs of the remaining
   // layers are built by subtree launches
-  steps.push_back({Step::LAYER, max_log, 1});
-  int32_t top = (int32_t)max_log - 1;
+  // a small tree whose leaves take one compression is built from its leaves by one
+  // subtree launch (no separate leaf launch)
+  const bool leaves_in_subtree = max_log <= TAIL_MAX_TOP && lcs[max_log].n_cols <= 16;
+  if (!leaves_in_subtree) steps.push_back({Step::LAYER, max_log, 1});
+  int32_t top = (int32_t)max_log - (leaves_in_subtree ? 0 : 1);
   while (top >= 0) {
     if (top > WIDE_LAYER || ((lcs[top].n_cols + 15) / 16 >= HEAVY_CHUNKS && top >= 8)) {
       steps.push_back({Step::LAYER, (uint32_t)top, 1});
@@ -511,7 +517,8 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
         drawn = true;
       }
       timed("merkle_subtree", [&] {
-        subtree_kernel<SUB_NT><<<blocks, SUB_NT, lds, st>>>(outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs, rd,
+        subtree_kernel<SUB_NT><<<blocks, SUB_NT, lds, st>>>(s.top == max_log ? nullptr : outs[s.top + 1], s.top,
+                                                            s.depth, w_log, douts, dlcs, rd,
                                                             tail ? darrive + s.top : nullptr);
       });
     }

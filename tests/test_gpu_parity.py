@@ -268,8 +268,11 @@ def test_commit_on_layer_with_previous():
     assert (got2.to_numpy().reshape(-1) == O.merkle_commit_on_layer(10, O.merkle_commit_on_layer(11, oprev, c11), [])).all()
 
 
+# trees of at most 2^12 leaves with at most 16 leaf columns are built from the leaves by
+# one subtree launch: [0], [3, 1, 0], [12] * 16, [12, 5, 12, ...]; [11] * 17 is not
 @pytest.mark.parametrize("sizes", [[16] * 16, [12, 5, 12, 3, 9, 9, 0, 12], [17, 17, 3], [13] * 33 + [2, 1],
-                                   [18] + [14] * 40 + [16] * 3 + [11] * 20 + [9]])
+                                   [18] + [14] * 40 + [16] * 3 + [11] * 20 + [9], [0], [3, 1, 0], [12] * 16,
+                                   [11] * 17 + [2]])
 def test_merkle_tree_mixed_degree(sizes):
     cols = [rand_m31(1 << s, 500 + i) for i, s in enumerate(sizes)]
     tree = S.MerkleTree([S.BaseFieldVec.from_numpy(c) for c in cols], sizes).commit_async()
