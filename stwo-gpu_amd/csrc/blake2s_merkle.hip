@@ -157,7 +157,9 @@ constexpr int QROT1 = 0x39, QROT2 = 0x4E, QROT3 = 0x93;  // lane q reads lane q+
   b = rotr(b ^ c, 7);
 
 // (h_lo, h_hi) = (h[q], h[4+q]) of the node's state; msg: the node's 16 message words in LDS.
-__device__ __forceinline__ void quad_compress(uint32_t& h_lo, uint32_t& h_hi, const uint32_t* msg, uint32_t q) {
+// dx is XORed into v[12 + q] (BLAKE2s counter and last-block flag; zero for tree nodes).
+__device__ __forceinline__ void quad_compress(uint32_t& h_lo, uint32_t& h_hi, const uint32_t* msg, uint32_t q,
+                                              uint32_t dx = 0) {
   const uint32_t sh = 8 * q;
   uint32_t m[40];
 #pragma unroll
@@ -168,7 +170,7 @@ __device__ __forceinline__ void quad_compress(uint32_t& h_lo, uint32_t& h_hi, co
                               0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
   uint32_t a = h_lo, b = h_hi;
   uint32_t c = q == 0 ? IV[0] : q == 1 ? IV[1] : q == 2 ? IV[2] : IV[3];
-  uint32_t d = q == 0 ? IV[4] : q == 1 ? IV[5] : q == 2 ? IV[6] : IV[7];
+  uint32_t d = (q == 0 ? IV[4] : q == 1 ? IV[5] : q == 2 ? IV[6] : IV[7]) ^ dx;
 #pragma unroll
   for (int r = 0; r < 10; r++) {
     QG(a, b, c, d, m[4 * r], m[4 * r + 1])
@@ -249,6 +251,69 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
   }
   out[i].s[q] = h_lo;
   out[i].s[4 + q] = h_hi;
+}
+
+// One BLAKE2s-256 block of the 16 words at msg (LDS) on a lane quad, for a message of
+// t_bytes bytes (blake2s256_block): returns (h[q], h[4+q]) of the digest.
+__device__ __forceinline__ void quad_blake2s256_block(const uint32_t* msg, uint32_t q, uint32_t t_bytes,
+                                                      uint32_t& h_lo, uint32_t& h_hi) {
+  constexpr uint32_t IVS[8] = {b2s::IV0, b2s::IV1, b2s::IV2, b2s::IV3, b2s::IV4, b2s::IV5, b2s::IV6, b2s::IV7};
+  h_lo = (q == 0 ? IVS[0] ^ 0x01010020u : q == 1 ? IVS[1] : q == 2 ? IVS[2] : IVS[3]);
+  h_hi = (q == 0 ? IVS[4] : q == 1 ? IVS[5] : q == 2 ? IVS[6] : IVS[7]);
+  quad_compress(h_lo, h_hi, msg, q, q == 0 ? t_bytes : q == 2 ? 0xFFFFFFFFu : 0u);
+}
+
+// The channel step after a commitment, on lanes 0..3 of a wave (one quad): digest <-
+// B2S(digest || root) (Channel::mix8), then draws B2S(digest || n || 0) for n = 0, 1, ...
+// until a QM31 (or, with rd.point, a circle point) is drawn, exactly as Channel::draw_qm31 /
+// draw_point; the compressions run four G functions side by side.  mb: 16 words of LDS
+// owned by the quad.
+__device__ __noinline__ void channel_step_quad(RootDraw rd, const uint32_t* root, uint32_t* mb, uint32_t q) {
+  mb[q] = rd.ch->digest[q];
+  mb[4 + q] = rd.ch->digest[4 + q];
+  mb[8 + q] = root[q];
+  mb[12 + q] = root[4 + q];
+  __builtin_amdgcn_wave_barrier();
+  uint32_t g_lo, g_hi;  // digest words q and 4 + q
+  quad_blake2s256_block(mb, q, 64, g_lo, g_hi);
+  uint32_t n = 0;
+  qm31 x = qzero(), y = qzero(), t = qzero();
+  for (;;) {
+    __builtin_amdgcn_wave_barrier();  // the quad's reads of the previous message are done
+    mb[q] = g_lo;
+    mb[4 + q] = g_hi;
+    mb[8 + q] = q == 0 ? n : 0u;
+    mb[12 + q] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t w_lo, w_hi;
+    quad_blake2s256_block(mb, q, 40, w_lo, w_hi);
+    n++;
+    __builtin_amdgcn_wave_barrier();
+    mb[q] = w_lo;  // words 0..3 of the draw
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t w0 = mb[0], w1 = mb[1], w2 = mb[2], w3 = mb[3];
+    if (!(w0 < 2 * P && w1 < 2 * P && w2 < 2 * P && w3 < 2 * P)) continue;  // uniform over the quad
+    t = qm31{{w0 >= P ? w0 - P : w0, w1 >= P ? w1 - P : w1}, {w2 >= P ? w2 - P : w2, w3 >= P ? w3 - P : w3}};
+    if (!rd.point) break;
+    const qm31 t2 = qmul(t, t);
+    const qm31 den = qadd(qone(), t2);
+    if (qis_zero(den)) continue;
+    const qm31 di = qinv(den);
+    x = qmul(qsub(qone(), t2), di);
+    y = qmul(qadd(t, t), di);
+    break;
+  }
+  rd.ch->digest[q] = g_lo;
+  rd.ch->digest[4 + q] = g_hi;
+  if (q == 0) {
+    rd.ch->n_draws = n;
+    if (rd.point) {
+      rd.dst[0] = x;
+      rd.dst[1] = y;
+    } else {
+      *rd.dst = t;
+    }
+  }
 }
 
 // Subtree levels: layers top, top-1, ..., top-depth+1 (each halving), built by one block.
@@ -368,16 +433,9 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
     const uint32_t t_top = top - bd;  // the block roots are layer t_top + 1
     build_levels<NT>(outs[t_top + 1], t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg);
   }
-  // the launch that wrote the root (layer 0, held in s[0] after the last barrier)
-  if (rd.ch && top + 1 == depth && threadIdx.x == 0) {
-    Channel ch = *rd.ch;
-    ch.mix8(s[0].s);
-    if (rd.point)
-      ch.draw_point(rd.dst[0], rd.dst[1]);
-    else
-      *rd.dst = ch.draw_qm31();
-    *rd.ch = ch;
-  }
+  // the launch that wrote the root (layer 0, held in s[0] after the last barrier) runs
+  // the channel step on one lane quad: Channel::mix8 and Channel::draw_qm31 / draw_point
+  if (rd.ch && top + 1 == depth && threadIdx.x < 4) channel_step_quad(rd, s[0].s, qmsg[0], threadIdx.x);
 }
 
 static uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
