@@ -259,10 +259,11 @@ static size_t slot_bytes(uint32_t n_slots) {
   return (sizeof(quot::ColSlot<1>) * (n_slots + quot::QG) + 255) & ~(size_t)255;
 }
 static constexpr size_t BATCH_BYTES = (sizeof(quot::Batch) + 255) & ~(size_t)255;
-// scratch of accumulate_quotients_single_point_async for domains up to log max_log:
-// slots | batch | domain tables
+// scratch of accumulate_quotients_single_point_async: slots | batch (the domain tables
+// come from the library's per-domain cache)
 size_t quotient_single_point_scratch_bytes(uint32_t n_cols, uint32_t max_log) {
-  return slot_bytes(padded_slots(std::max<uint32_t>(n_cols, 1))) + BATCH_BYTES + quot::dt_bytes(max_log - 1);
+  (void)max_log;
+  return slot_bytes(padded_slots(std::max<uint32_t>(n_cols, 1))) + BATCH_BYTES;
 }
 
 // Fill the domain tables of CircleDomain(half coset (hi, hs), log log_size) at `dev`
@@ -307,7 +308,7 @@ void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t 
   const uint32_t n_slots = padded_slots(n_cols);
   quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
   quot::Batch* bs = (quot::Batch*)((char*)d_scratch + slot_bytes(n_slots));
-  const quot::DomainTables T = domain_tables_async(hi, hs, log_size, (char*)bs + BATCH_BYTES);
+  const quot::DomainTables T = cached_domain_tables(hi, hs, log_size);
   timed("quotient_constants", [&] {
     quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss,
                                                                 n_slots + quot::QG, bs);
