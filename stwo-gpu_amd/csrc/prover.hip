@@ -31,6 +31,7 @@
 #include <cstring>
 #include <map>
 
+#include "blake2s_quad.hpp"
 #include "channel.hpp"
 #include "fft.hpp"
 #include "prover.hpp"
@@ -104,89 +105,106 @@ __global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root) {
   jb->ch.draw_point(jb->z[0], jb->z[1]);
 }
 // Tree root (blake2s.hpp) of a QM31 vector held as AoS (soa == false: v[i] = 4 words) or as
-// four coordinate columns, by one block: leaves from global memory, levels in LDS.
+// four coordinate columns, by one block: one lane quad per node (quad_compress), leaves
+// from global memory into buf, levels alternating between buf2 and buf; returns the root's
+// LDS slot.  qm: 16 LDS words per quad.
 constexpr uint32_t TREE_NT = 256;
+constexpr uint32_t TREE_NQ = TREE_NT / 4;
 constexpr uint32_t TREE_MAX_LEAVES = 1040;  // 4 words per element: up to 4160 elements
 struct Soa4c {
   const uint32_t* c[4];
 };
-__device__ void block_tree_root(const qm31* aos, Soa4c soa, bool use_soa, uint32_t n, uint32_t (&root)[8]) {
-  __shared__ merkle::Hash buf[TREE_MAX_LEAVES];
-  const uint32_t n_words = 4 * n, nb = (n_words + 15) / 16;
-  for (uint32_t i = threadIdx.x; i < nb; i += TREE_NT) {
-    uint32_t m[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const uint32_t w = 16 * i + j, e = w >> 2, k = w & 3;
-      uint32_t x = 0;
-      if (w < n_words) {
-        if (use_soa) {
-          x = soa.c[k][e];
-        } else {
-          const qm31 v = aos[e];
-          x = k == 0 ? v.x.a : k == 1 ? v.x.b : k == 2 ? v.y.a : v.y.b;
-        }
+__device__ const merkle::Hash* block_tree_root(const qm31* aos, Soa4c soa, bool use_soa, uint32_t n,
+                                               merkle::Hash* buf, merkle::Hash* buf2, uint32_t (*qm)[16]) {
+  const uint32_t nb = (4 * n + 15) / 16;
+  const uint32_t qi = threadIdx.x >> 2, q = threadIdx.x & 3;
+  uint32_t* mb = qm[qi];
+  // leaf i holds elements 4i .. 4i+3; lane q stages element 4i + q (words 4q .. 4q+3)
+#pragma unroll 1
+  for (uint32_t i = qi; i < nb; i += TREE_NQ) {  // uniform over the quad
+    const uint32_t e = 4 * i + q;
+    uint32_t x[4] = {0, 0, 0, 0};
+    if (e < n) {
+      if (use_soa) {
+        for (int k = 0; k < 4; k++) x[k] = soa.c[k][e];
+      } else {
+        const qm31 v = aos[e];
+        x[0] = v.x.a;
+        x[1] = v.x.b;
+        x[2] = v.y.a;
+        x[3] = v.y.b;
       }
-      m[j] = x;
     }
-    uint32_t h[8];
-    b2s::f0(m, h);
-#pragma unroll
-    for (int j = 0; j < 8; j++) buf[i].s[j] = h[j];
+    __builtin_amdgcn_wave_barrier();  // the quad's reads of its previous message are done
+    for (int k = 0; k < 4; k++) mb[4 * q + k] = x[k];
+    __builtin_amdgcn_wave_barrier();
+    uint32_t h_lo = 0, h_hi = 0;
+    merkle::quad_compress(h_lo, h_hi, mb, q);
+    buf[i].s[q] = h_lo;
+    buf[i].s[4 + q] = h_hi;
   }
   __syncthreads();
-  // each level: nodes are computed into registers (at most PAIRS per lane), then written
-  // back after a barrier, so no lane overwrites a node another lane still reads
-  constexpr uint32_t PAIRS = (TREE_MAX_LEAVES / 2 + TREE_NT - 1) / TREE_NT;
+  // each level reads one buffer and writes the other (an unpaired last node moves up as is)
+  merkle::Hash* in = buf;
+  merkle::Hash* out = buf2;
   uint32_t cnt = nb;
   while (cnt > 1) {
     const uint32_t half = (cnt + 1) / 2;
-    uint32_t nodes[PAIRS][8];
-#pragma unroll
-    for (uint32_t k = 0; k < PAIRS; k++) {
-      const uint32_t i = threadIdx.x + k * TREE_NT;
-      if (i >= half) break;
+#pragma unroll 1
+    for (uint32_t i = qi; i < half; i += TREE_NQ) {  // uniform over the quad
+      uint32_t lo = 0, hi = 0;
       if (2 * i + 1 < cnt) {
-        uint32_t m[16];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          m[j] = buf[2 * i].s[j];
-          m[8 + j] = buf[2 * i + 1].s[j];
-        }
-        b2s::f0(m, nodes[k]);
+        merkle::quad_compress(lo, hi, in[2 * i].s, q);  // children 2i, 2i+1: 16 words
       } else {
-#pragma unroll
-        for (int j = 0; j < 8; j++) nodes[k][j] = buf[2 * i].s[j];
+        lo = in[2 * i].s[q];
+        hi = in[2 * i].s[4 + q];
       }
+      out[i].s[q] = lo;
+      out[i].s[4 + q] = hi;
     }
     __syncthreads();
-#pragma unroll
-    for (uint32_t k = 0; k < PAIRS; k++) {
-      const uint32_t i = threadIdx.x + k * TREE_NT;
-      if (i >= half) break;
-#pragma unroll
-      for (int j = 0; j < 8; j++) buf[i].s[j] = nodes[k][j];
-    }
-    __syncthreads();
+    merkle::Hash* t = in;
+    in = out;
+    out = t;
     cnt = half;
   }
-#pragma unroll
-  for (int j = 0; j < 8; j++) root[j] = buf[0].s[j];
+  return in;
 }
 
+// mix the tree root of the OODS samples, draw the quotient coefficient and the first FRI
+// folding coefficient (Channel::mix8, two Channel::draw_qm31), on one quad
 __global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n) {
-  uint32_t root[8];
-  block_tree_root(samples, Soa4c{}, false, n, root);
-  if (threadIdx.x == 0) {
-    jb->ch.mix8(root);
-    jb->beta = jb->ch.draw_qm31();
-    jb->fri_alpha[0] = jb->ch.draw_qm31();
+  __shared__ merkle::Hash buf[TREE_MAX_LEAVES], buf2[TREE_MAX_LEAVES / 2];
+  __shared__ uint32_t qm[TREE_NQ][16];
+  const merkle::Hash* root = block_tree_root(samples, Soa4c{}, false, n, buf, buf2, qm);
+  if (threadIdx.x < 4) {
+    const uint32_t q = threadIdx.x;
+    uint32_t g_lo, g_hi, k = 0;
+    merkle::quad_mix8(&jb->ch, root->s, qm[0], q, g_lo, g_hi);
+    const qm31 beta = merkle::quad_draw_qm31(g_lo, g_hi, k, qm[0], q);
+    const qm31 alpha0 = merkle::quad_draw_qm31(g_lo, g_hi, k, qm[0], q);
+    jb->ch.digest[q] = g_lo;
+    jb->ch.digest[4 + q] = g_hi;
+    if (q == 0) {
+      jb->ch.n_draws = k;
+      jb->beta = beta;
+      jb->fri_alpha[0] = alpha0;
+    }
   }
 }
+// mix the tree root of the last FRI layer (Channel::mix8), on one quad
 __global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
-  uint32_t root[8];
-  block_tree_root(nullptr, last, true, n, root);
-  if (threadIdx.x == 0) jb->ch.mix8(root);
+  __shared__ merkle::Hash buf[TREE_MAX_LEAVES], buf2[TREE_MAX_LEAVES / 2];
+  __shared__ uint32_t qm[TREE_NQ][16];
+  const merkle::Hash* root = block_tree_root(nullptr, last, true, n, buf, buf2, qm);
+  if (threadIdx.x < 4) {
+    const uint32_t q = threadIdx.x;
+    uint32_t g_lo, g_hi;
+    merkle::quad_mix8(&jb->ch, root->s, qm[0], q, g_lo, g_hi);
+    jb->ch.digest[q] = g_lo;
+    jb->ch.digest[4 + q] = g_hi;
+    if (q == 0) jb->ch.n_draws = 0;
+  }
 }
 __global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size) {
   const uint32_t w[2] = {(uint32_t)jb->nonce, (uint32_t)(jb->nonce >> 32)};
