@@ -116,6 +116,15 @@ __host__ __device__ __forceinline__ void blake2s256_block(const uint32_t* words,
   for (int i = 0; i < 8; i++) out[i] = h[i];
 }
 
+// BLAKE2s-256 of a message of n_words <= 16 words given zero-padded to one block.
+__host__ __device__ __forceinline__ void blake2s256_padded_block(const uint32_t (&m)[16], uint32_t n_words,
+                                                                uint32_t (&out)[8]) {
+  uint32_t h[8] = {IV0 ^ 0x01010020u, IV1, IV2, IV3, IV4, IV5, IV6, IV7};
+  compress(h, m, 4ull * n_words, true);
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i] = h[i];
+}
+
 // BLAKE2s-256 of a message given as little-endian 32-bit words (n_words * 4 bytes).
 __host__ __device__ inline void blake2s256_words(const uint32_t* words, uint32_t n_words, uint32_t* out) {
   Hasher hs;

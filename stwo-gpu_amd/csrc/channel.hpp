@@ -27,6 +27,14 @@ struct Channel {
     n_draws = 0;
   }
   __host__ __device__ void mix(const uint32_t* w, uint32_t n) {
+    if (n <= 8) {  // digest || w is one block: message and state in registers
+      uint32_t m[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) m[i] = i < 8 ? digest[i] : ((uint32_t)(i - 8) < n ? w[i - 8] : 0u);
+      b2s::blake2s256_padded_block(m, 8 + n, digest);
+      n_draws = 0;
+      return;
+    }
     b2s::Hasher hs;
     hs.words(digest, 8);
     hs.words(w, n);
