@@ -68,6 +68,7 @@ def _declare(L):
         "or_accumulate": (None, [u64, C.POINTER(u32p), C.POINTER(u32p)]),
         "or_wide_fib_constraint_quotients": (None, [u64, u32, C.POINTER(u32p), u32p, u32p,
                                                     C.POINTER(u32p)]),
+        "or_set_mutation": (None, [C.c_int]),
         "or_uniform_m31": (u32, [u64, u64]),
         "or_fill_uniform_m31": (None, [u32p, u64, u64, u64]),
     }
@@ -305,6 +306,25 @@ def wide_fib_constraint_quotients(trace, coeff_powers, denom_inv):
     out = [np.zeros(n, np.uint32) for _ in range(4)]
     lib().or_wide_fib_constraint_quotients(n, len(trace), _ptrs(trace), _p(cp), _p(di), _ptrs(out))
     return np.stack(out)
+
+
+# ---------------------------------------------------------------- planted mistakes
+MUT_QUOT_ALPHA_POW, MUT_QUOT_BATCH_POW, MUT_QUOT_BATCH_ORDER, MUT_QUOT_REF_OFFSET, \
+    MUT_WF_POWER_ORDER = 1, 2, 3, 4, 5
+
+
+class mutation:
+    """Context manager for the oracle's pinning tests: inside it the oracle
+    computes the named deliberately wrong variant (stwo_oracle.h OR_MUT_*)."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
 
 
 # ---------------------------------------------------------------- inputs

@@ -20,6 +20,13 @@
 
 #define P OR_P
 
+/* Planted mistakes for the oracle's own pinning tests (tests/test_oracle.py
+   sets one, checks that a definition-derived check rejects the result, and
+   resets it to 0).  Every other caller leaves it at 0.  See OR_MUT_* in
+   stwo_oracle.h. */
+static int g_mutation = OR_MUT_NONE;
+void or_set_mutation(int m) { g_mutation = m; }
+
 /* ------------------------------------------------------------------------- */
 /* M31: cuda/src/fields.cu:3-40                                               */
 /* ------------------------------------------------------------------------- */
@@ -501,11 +508,11 @@ void or_accumulate_quotients(uint32_t hi, uint32_t hs, uint32_t log_size,
     for (uint32_t b = 0; b < n_batches; b++) {
         qm31 py = qm_load(points + 8 * b + 4);
         qm31 py_conj = {py.x, cm_neg(py.y)};
-        bc[b] = qm_pow(rc, sizes[b]);
+        bc[b] = qm_pow(rc, sizes[b] + (g_mutation == OR_MUT_QUOT_BATCH_POW ? 1 : 0));
         qm31 alpha = qm_from_m31(1);
         for (uint32_t j = 0; j < sizes[b]; j++) {
             qm31 v = qm_load(values + 4 * (off + j));
-            alpha = qm_mul(alpha, rc);
+            if (g_mutation != OR_MUT_QUOT_ALPHA_POW) alpha = qm_mul(alpha, rc);
             qm31 v_conj = {v.x, cm_neg(v.y)};
             qm31 a = qm_sub(v_conj, v);
             qm31 c = qm_sub(py_conj, py);
@@ -513,8 +520,20 @@ void or_accumulate_quotients(uint32_t hi, uint32_t hs, uint32_t log_size,
             lc[3 * (off + j) + 0] = qm_mul(alpha, a);
             lc[3 * (off + j) + 1] = qm_mul(alpha, bb);
             lc[3 * (off + j) + 2] = qm_mul(alpha, c);
+            if (g_mutation == OR_MUT_QUOT_ALPHA_POW) alpha = qm_mul(alpha, rc);
         }
         off += sizes[b];
+    }
+    if (g_mutation == OR_MUT_QUOT_REF_OFFSET) {
+        /* the per-batch offset b * |b| * 3 of quotients.cu:74 (with the
+           cumulative read offset), which is wrong when batch sizes differ */
+        qm31 *lc2 = (qm31 *)malloc(sizeof(qm31) * 3 * (total ? total : 1));
+        memcpy(lc2, lc, sizeof(qm31) * 3 * (total ? total : 1));
+        for (uint32_t b = 0, o = 0; b < n_batches; o += sizes[b], b++)
+            for (uint32_t k = 0; k < 3 * sizes[b]; k++)
+                if (3 * b * sizes[b] + k < 3 * total) lc2[3 * b * sizes[b] + k] = lc[3 * o + k];
+        free(lc);
+        lc = lc2;
     }
     int64_t n = 1ll << log_size;
 #pragma omp parallel for schedule(static)
@@ -522,7 +541,10 @@ void or_accumulate_quotients(uint32_t hi, uint32_t hs, uint32_t log_size,
         pt dp = pt_at_index(circle_domain_index(hi, hs, log_size - 1, brev((uint64_t)row, log_size)));
         qm31 acc = qm_from_m31(0);
         uint32_t o = 0;
-        for (uint32_t b = 0; b < n_batches; b++) {
+        if (g_mutation == OR_MUT_QUOT_BATCH_ORDER) o = total;
+        for (uint32_t bi = 0; bi < n_batches; bi++) {
+            uint32_t b = g_mutation == OR_MUT_QUOT_BATCH_ORDER ? n_batches - 1 - bi : bi;
+            if (g_mutation == OR_MUT_QUOT_BATCH_ORDER) o -= sizes[b];
             /* denominator (Pr.x - p.x) Pi.y - (Pr.y - p.y) Pi.x in CM31 (quotients.cu:91-109) */
             cm31 prx = {points[8 * b + 0], points[8 * b + 1]}, pix = {points[8 * b + 2], points[8 * b + 3]};
             cm31 pry = {points[8 * b + 4], points[8 * b + 5]}, piy = {points[8 * b + 6], points[8 * b + 7]};
@@ -536,7 +558,7 @@ void or_accumulate_quotients(uint32_t hi, uint32_t hs, uint32_t log_size,
                 num = qm_add(num, qm_sub(qm_scale(c, cv), qm_add(qm_scale(a, dp.y), bb)));
             }
             acc = qm_add(qm_mul(acc, bc[b]), qm_mul_cm(num, dinv));
-            o += sizes[b];
+            if (g_mutation != OR_MUT_QUOT_BATCH_ORDER) o += sizes[b];
         }
         qm_store_soa(acc, out, (uint64_t)row);
     }
@@ -562,7 +584,8 @@ void or_wide_fib_constraint_quotients(uint64_t n_rows, uint32_t n_cols,
         for (uint32_t i = 0; i + 2 < n_cols; i++) {
             uint32_t a = trace[i][r], b = trace[i + 1][r], c = trace[i + 2][r];
             uint32_t cst = or_m31_sub(or_m31_add(or_m31_mul(a, a), or_m31_mul(b, b)), c);
-            num = qm_add(num, qm_scale(qm_load(coeff_powers + 4 * (n_cols - 3 - i)), cst));
+            uint32_t k = g_mutation == OR_MUT_WF_POWER_ORDER ? i : n_cols - 3 - i;
+            num = qm_add(num, qm_scale(qm_load(coeff_powers + 4 * k), cst));
         }
         qm_store_soa(qm_scale(num, denom_inv[r]), out, (uint64_t)r);
     }

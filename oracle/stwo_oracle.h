@@ -115,6 +115,20 @@ void or_wide_fib_constraint_quotients(uint64_t n_rows, uint32_t n_cols,
                                       const uint32_t *coeff_powers,
                                       const uint32_t *denom_inv, uint32_t *const out[4]);
 
+/* ---- planted mistakes, for the oracle's pinning tests only ----
+ * or_set_mutation(m) makes later calls compute a deliberately wrong variant;
+ * 0 (OR_MUT_NONE) restores the restatement.  Nothing but tests/test_oracle.py
+ * sets it. */
+enum {
+    OR_MUT_NONE = 0,
+    OR_MUT_QUOT_ALPHA_POW = 1,   /* sample j of a batch weighted alpha^j, not alpha^(j+1) */
+    OR_MUT_QUOT_BATCH_POW = 2,   /* batch chain coefficient rc^(|b|+1), not rc^|b| */
+    OR_MUT_QUOT_BATCH_ORDER = 3, /* batches accumulated in reverse order */
+    OR_MUT_QUOT_REF_OFFSET = 4,  /* line coefficients stored at b*|b|*3 (quotients.cu:74) */
+    OR_MUT_WF_POWER_ORDER = 5,   /* constraint i weighted coeff_pow[i], not coeff_pow[n-3-i] */
+};
+void or_set_mutation(int m);
+
 /* ---- synthetic inputs (counter-based, identical to the device generator) ---- */
 uint32_t or_uniform_m31(uint64_t seed, uint64_t index);
 void or_fill_uniform_m31(uint32_t *dst, uint64_t n, uint64_t seed, uint64_t offset);

@@ -167,3 +167,36 @@ def random_qm31_point(rng):
             continue
         inv = qm_inv(den)
         return qm_mul(qm_sub(qm(1), t2), inv), qm_mul(qm_scale(t, 2), inv)
+
+
+# ---------------------------------------------------------------- DEEP quotient pieces
+def qm_conj(p):
+    """Conjugation over CM31: x + y u -> x - y u."""
+    return (p[0], p[1], (-p[2]) % P, (-p[3]) % P)
+
+
+def qm_div(p, q): return qm_mul(p, qm_inv(q))
+
+
+U = (0, 0, 1, 0)
+
+
+def pair_vanishing(e0, e1, p):
+    """Line through the circle points e0 and e1, as the determinant
+        | p.x  p.y  1 |
+        | e0.x e0.y 1 |
+        | e1.x e1.y 1 |
+    (zero exactly on the line through e0 and e1); every argument is a
+    QM31 point ((x4), (y4))."""
+    px, py = p
+    t0 = qm_mul(px, qm_sub(e0[1], e1[1]))
+    t1 = qm_mul(py, qm_sub(e0[0], e1[0]))
+    t2 = qm_sub(qm_mul(e0[0], e1[1]), qm_mul(e0[1], e1[0]))
+    return qm_add(qm_sub(t0, t1), t2)
+
+
+def conj_line(point, value, y):
+    """The line through (P.y, v) and (conj P.y, conj v), evaluated at y."""
+    c = qm_sub(qm_conj(point[1]), point[1])
+    slope = qm_div(qm_sub(qm_conj(value), value), c)
+    return qm_add(value, qm_mul(slope, qm_sub(y, point[1])))

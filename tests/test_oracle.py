@@ -350,6 +350,126 @@ def test_quotients_low_degree_and_broken_sample():
     assert not _interp_low_degree(q2, m, hc[0], itw)
 
 
+# The DEEP quotient of stwo's complex-conjugate line method, written from its
+# definition rather than from the restatement's code: for domain point p,
+#   Q(p) = sum_b W_b * sum_j rc^(j+1) * c_b * (f_bj(p) - L_bj(p.y)) / D_b(p)
+# with c_b = conj(P_b.y) - P_b.y, L_bj the line through (P_b.y, v_bj) and its
+# conjugate, D_b(p) the pair vanishing of P_b and conj(P_b) at p divided by -2u
+# (the CM31 value quotients.cu:91-109 inverts), W_b = rc^(sum of |b'| over the
+# batches after b) (the batch chain acc <- acc * rc^|b| + ..., quotients.cu:70,167)
+# and alpha_j = rc^(j+1) within a batch (quotients.cu:76-80).
+def _deep_denominator(point, p):
+    d = R.qm_div(R.pair_vanishing(point, (R.qm_conj(point[0]), R.qm_conj(point[1])), (R.qm(p[0]), R.qm(p[1]))),
+                 R.qm_scale(R.U, P - 2))
+    assert d[2] == 0 and d[3] == 0, "the denominator lies in CM31"
+    return d
+
+
+def _deep_quotient_row(p, row, ldes, rc, batches):
+    total = R.qm(0)
+    for b, (point, cv) in enumerate(batches):
+        w = R.qm_pow(rc, sum(len(c) for _, c in batches[b + 1:]))
+        c_b = R.qm_sub(R.qm_conj(point[1]), point[1])
+        dinv = R.qm_inv(_deep_denominator(point, p))
+        for j, (col, v) in enumerate(cv):
+            f = R.qm(int(ldes[col][row]))
+            t = R.qm_mul(c_b, R.qm_sub(f, R.conj_line(point, v, R.qm(p[1]))))
+            total = R.qm_add(total, R.qm_mul(R.qm_mul(w, R.qm_pow(rc, j + 1)), R.qm_mul(t, dinv)))
+    return total
+
+
+def _quotient_case(seed, layout, log_n=3, blow=1, true_samples=False):
+    """layout: per batch, the list of sampled column indices."""
+    rng = random.Random(seed)
+    m = log_n + blow
+    hc = R.canonic_half_coset(m)
+    tw, itw = O.precompute_twiddles(*hc)
+    n_cols = 1 + max(c for cols in layout for c in cols)
+    nrng = np.random.default_rng(seed)
+    trace = [nrng.integers(0, P, 1 << log_n, dtype=np.uint32) for _ in range(n_cols)]
+    coeffs, ldes = O.lde_columns(trace, blow, tw, itw)
+    batches = []
+    for cols in layout:
+        pt = R.random_qm31_point(rng)
+        vals = [O.eval_at_point(coeffs[c], pt[0], pt[1]) if true_samples else
+                tuple(rng.randrange(P) for _ in range(4)) for c in cols]
+        batches.append((pt, list(zip(cols, vals))))
+    rc = tuple(rng.randrange(P) for _ in range(4))
+    pts = R.circle_domain_points(*hc)
+    dom = [pts[R.brev(r, m)] for r in range(1 << m)]
+    return hc, m, ldes, rc, batches, dom
+
+
+def _quotients(case):
+    hc, m, ldes, rc, batches, _ = case
+    return O.accumulate_quotients(hc[0], hc[1], m, ldes, rc, batches)
+
+
+def _matches_definition(q, case, rows=None):
+    _, m, ldes, rc, batches, dom = case
+    for r in rows if rows is not None else range(1 << m):
+        if tuple(int(v) for v in q[:, r]) != _deep_quotient_row(dom[r], r, ldes, rc, batches):
+            return False
+    return True
+
+
+QUOT_LAYOUTS = {
+    "one-sample": [[0]],
+    "one-batch": [[0, 2, 1]],
+    "uneven": [[0, 1, 2], [1], [2, 0]],
+    "four": [[1, 0], [3], [0, 1, 2, 3], [2]],
+}
+
+
+@pytest.mark.parametrize("name", sorted(QUOT_LAYOUTS))
+@pytest.mark.parametrize("true_samples", [False, True])
+def test_quotients_match_definition(name, true_samples):
+    case = _quotient_case(11, QUOT_LAYOUTS[name], true_samples=true_samples)
+    assert _matches_definition(_quotients(case), case)
+
+
+def _perturbation_holds(case):
+    # Moving one sampled value v_bj by delta changes every row by exactly
+    #   -W_b rc^(j+1) c_b (delta + (conj delta - delta)(p.y - P_b.y)/c_b) / D_b(p).
+    hc, m, ldes, rc, batches, dom = case
+    rng = random.Random(5)
+    for b, j in ((0, 2), (1, 0), (2, 1)):
+        delta = tuple(rng.randrange(P) for _ in range(4))
+        bad = [(pt, list(cv)) for pt, cv in batches]
+        col, v = bad[b][1][j]
+        bad[b][1][j] = (col, R.qm_add(v, delta))
+        diff = O.accumulate_quotients(hc[0], hc[1], m, ldes, rc, bad).astype(np.int64) - _quotients(case)
+        point = batches[b][0]
+        c_b = R.qm_sub(R.qm_conj(point[1]), point[1])
+        w = R.qm_mul(R.qm_pow(rc, sum(len(c) for _, c in batches[b + 1:])), R.qm_pow(rc, j + 1))
+        for r in range(1 << m):
+            y = R.qm(dom[r][1])
+            dl = R.qm_add(delta, R.qm_div(R.qm_mul(R.qm_sub(R.qm_conj(delta), delta), R.qm_sub(y, point[1])), c_b))
+            want = R.qm_sub(R.qm(0), R.qm_mul(w, R.qm_div(R.qm_mul(c_b, dl), _deep_denominator(point, dom[r]))))
+            if tuple(int(d) % P for d in diff[:, r]) != want:
+                return False
+    return True
+
+
+def test_quotients_sample_perturbation():
+    assert _perturbation_holds(_quotient_case(12, QUOT_LAYOUTS["uneven"]))
+
+
+@pytest.mark.parametrize("mut", [O.MUT_QUOT_ALPHA_POW, O.MUT_QUOT_BATCH_POW, O.MUT_QUOT_BATCH_ORDER,
+                                 O.MUT_QUOT_REF_OFFSET])
+def test_quotient_checks_reject_planted_mistakes(mut):
+    # Each planted coefficient mistake must fail the definition check; the
+    # low-degree check alone would pass the first three with true samples.
+    case = _quotient_case(13, QUOT_LAYOUTS["uneven"], true_samples=True)
+    with O.mutation(mut):
+        q = _quotients(case)
+    assert not _matches_definition(q, case, rows=range(4))
+    assert _matches_definition(_quotients(case), case, rows=range(4))
+    if mut != O.MUT_QUOT_REF_OFFSET:  # a storage-offset mistake may keep differences linear
+        with O.mutation(mut):
+            assert not _perturbation_holds(case)
+
+
 def test_accumulate_closed_form():
     # accumulation.rs:31-101: (1,2,3,4)+(5,6,7,8) and 1 + (P-1) = 0
     n = 64
@@ -400,6 +520,68 @@ def test_wide_fib_quotient_low_degree():
     bad = [c.copy() for c in trace]
     bad[5][3] = (int(bad[5][3]) + 1) % P
     assert not quotient_degree_ok(bad)
+
+
+# The composition numerator is stwo's accumulator chain over the constraints
+# in order, acc <- acc * alpha + C_i (PointEvaluationAccumulator: the verifier
+# evaluates the composition at the OODS point this way), so constraint i
+# carries alpha^(n-3-i); with powers [1, alpha, ...] that is coeff_pow[n-3-i]
+# (examples/wide_fibonacci.cu:13-27).
+def _wf_case(seed, n_cols=6, n=8):
+    rng = np.random.default_rng(seed)
+    trace = [rng.integers(0, P, n, dtype=np.uint32) for _ in range(n_cols)]
+    dinv = rng.integers(1, P, n, dtype=np.uint32)
+    alpha = tuple(int(v) for v in rng.integers(0, P, 4))
+    powers = np.array([R.qm_pow(alpha, k) for k in range(n_cols - 2)], np.uint32)
+    return trace, dinv, alpha, powers
+
+
+def _wf_constraint(trace, i, r):
+    a, b, c = (int(trace[i + k][r]) for k in range(3))
+    return (a * a + b * b - c) % P
+
+
+def _wf_horner(trace, dinv, alpha, r):
+    acc = R.qm(0)
+    for i in range(len(trace) - 2):
+        acc = R.qm_add(R.qm_mul(acc, alpha), R.qm(_wf_constraint(trace, i, r)))
+    return R.qm_scale(acc, int(dinv[r]))
+
+
+def _wf_matches(out, trace, dinv, alpha):
+    return all(tuple(int(v) for v in out[:, r]) == _wf_horner(trace, dinv, alpha, r)
+               for r in range(trace[0].size))
+
+
+def test_wide_fib_composition_matches_accumulator_chain():
+    trace, dinv, alpha, powers = _wf_case(21)
+    assert _wf_matches(O.wide_fib_constraint_quotients(trace, powers, dinv), trace, dinv, alpha)
+
+
+def test_wide_fib_composition_perturbation():
+    # Raising t_k at row r by d changes that row, and only that row, by
+    #   dinv * sum over the constraints i that read t_k of alpha^(n-3-i) * dC_i.
+    trace, dinv, alpha, powers = _wf_case(22)
+    n_cols = len(trace)
+    base = O.wide_fib_constraint_quotients(trace, powers, dinv)
+    for k, r, d in ((n_cols - 1, 3, 5), (2, 0, 1), (4, 6, 1234567)):
+        bad = [c.copy() for c in trace]
+        bad[k][r] = (int(bad[k][r]) + d) % P
+        diff = (O.wide_fib_constraint_quotients(bad, powers, dinv).astype(np.int64) - base) % P
+        want = R.qm(0)
+        for i in range(n_cols - 2):
+            if i <= k <= i + 2:
+                dc = (_wf_constraint(bad, i, r) - _wf_constraint(trace, i, r)) % P
+                want = R.qm_add(want, R.qm_scale(R.qm_pow(alpha, n_cols - 3 - i), dc))
+        assert tuple(int(v) for v in diff[:, r]) == R.qm_scale(want, int(dinv[r]))
+        assert not np.delete(diff, r, axis=1).any()
+
+
+def test_wide_fib_check_rejects_planted_power_order():
+    trace, dinv, alpha, powers = _wf_case(23)
+    with O.mutation(O.MUT_WF_POWER_ORDER):
+        out = O.wide_fib_constraint_quotients(trace, powers, dinv)
+    assert not _wf_matches(out, trace, dinv, alpha)
 
 
 # ------------------------------------------------------------------ inputs
