@@ -18,7 +18,11 @@
  *     printed to stderr once, recorded, and readable with stwo_hip_last_error();
  *     pointer-returning functions return NULL on failure.
  *   - Differences from the reference (all are fixes): every output is canonical
- *     (the reference's neg(0) = P); sizes up to 2^31 elements; evaluate_columns
+ *     (the reference's neg(0) = P); 64-bit sizes: bit reversal, batch inversion and
+ *     the folds take up to 2^31 elements, while columns that the FFT, Merkle,
+ *     quotient, OODS and composition kernels read hold at most 2^30 values (their
+ *     kernels form byte offsets in 32 bits; larger calls fail with an argument
+ *     error before any launch); evaluate_columns
  *     accepts columns of different sizes; accumulate_quotients uses the
  *     cumulative line-coefficient offset; interpolate/evaluate handle circle
  *     domains of log size 1 and 2 (assumed canonic, as stwo constructs them).
@@ -153,6 +157,20 @@ void accumulate_quotients(uint32_t half_coset_initial_index, uint32_t half_coset
                           uint32_t *result_column_2, uint32_t *result_column_3,
                           uint32_t flattened_line_coeffs_size);
 
+/* bindings.rs:238-244 -> cuda/src/gkr.cu:20-38 (GkrOps::gen_eq_evals, lookups/gkr.rs:12-27):
+ * evals[x] = v * prod_k (y_k if bit (y_size-1-k) of x else 1 - y_k), x < 2^y_size.
+ * y: HOST array of y_size QM31; evals: device QM31 (AoS) of evals_size = 2^y_size. */
+void gen_eq_evals(StwoQM31 v, const StwoQM31 *y, uint32_t y_size, StwoQM31 *evals, uint32_t evals_size);
+/* bindings.rs:246-251 -> cuda/src/mle.cu:14-20 (MleOps<BaseField>::fix_first_variable,
+ * lookups/mle.rs:12-31): output_evals[i] = lhs + (rhs - lhs) assignment with lhs = evals[i],
+ * rhs = evals[i + evals_size/2]; M31 in, QM31 (AoS) out of evals_size/2 elements. */
+void fix_first_variable_base_field(const uint32_t *evals, size_t evals_size, StwoQM31 assignment,
+                                   uint32_t *output_evals);
+/* bindings.rs:253-258 -> mle.cu:22-28 (MleOps<SecureField>::fix_first_variable, mle.rs:33-57);
+ * QM31 (AoS) in and out. */
+void fix_first_variable_secure_field(const uint32_t *evals, size_t evals_size, StwoQM31 assignment,
+                                     uint32_t *output_evals);
+
 /* bindings.rs:260-270 -> cuda/src/examples/wide_fibonacci.cu:40-65 (WideFib composition).
  * trace_evaluations: HOST array of number_of_columns device pointers; random_coeff_powers:
  * device QM31 (AoS) of number_of_columns - 2; denominator_inverses: device M31. */
@@ -273,8 +291,11 @@ int stwo_hip_accumulate_quotients_async(uint32_t half_coset_initial_index, uint3
  * it; autotune = 1 times the candidate FFT pass schedules of every size class once and
  * keeps the fastest (process-wide, per log size, blowup and column-count bucket), then
  * times graph replay against stream launches of the plan: where the streams are faster
- * run(1) launches from the streams too (uses_graph reports the choice).  No counterpart
- * in the reference (its commit is one synchronous call per operation). */
+ * run(1) launches from the streams too (uses_graph reports the choice).  Where coeffs[i]
+ * aliases evals[i], every run interpolates that column in place (after a run the buffer
+ * holds coefficients, so write the evaluations again before the next run); the
+ * autotuner's runs keep a copy of such columns and restore them before create returns.
+ * No counterpart in the reference (its commit is one synchronous call per operation). */
 typedef struct StwoCommitPlan StwoCommitPlan;
 int stwo_hip_commit_plan_create(StwoCommitPlan **plan, uint32_t n_cols, const uint32_t *const *evals,
                                 uint32_t *const *coeffs, uint32_t *const *ldes, const uint32_t *log_sizes,

@@ -68,6 +68,9 @@ def _declare(L):
         "or_accumulate": (None, [u64, C.POINTER(u32p), C.POINTER(u32p)]),
         "or_wide_fib_constraint_quotients": (None, [u64, u32, C.POINTER(u32p), u32p, u32p,
                                                     C.POINTER(u32p)]),
+        "or_gen_eq_evals": (None, [u32p, u32p, u32, u32p]),
+        "or_fix_first_variable_base": (None, [u32p, u64, u32p, u32p]),
+        "or_fix_first_variable_secure": (None, [u32p, u64, u32p, u32p]),
         "or_set_mutation": (None, [C.c_int]),
         "or_uniform_m31": (u32, [u64, u64]),
         "or_fill_uniform_m31": (None, [u32p, u64, u64, u64]),
@@ -306,6 +309,26 @@ def wide_fib_constraint_quotients(trace, coeff_powers, denom_inv):
     out = [np.zeros(n, np.uint32) for _ in range(4)]
     lib().or_wide_fib_constraint_quotients(n, len(trace), _ptrs(trace), _p(cp), _p(di), _ptrs(out))
     return np.stack(out)
+
+
+# ---------------------------------------------------------------- multilinear ops
+def gen_eq_evals(y, v):
+    """y: list of QM31 4-tuples; returns (2^len(y), 4) AoS evaluations."""
+    n = len(y)
+    yy = _arr([list(t) for t in y] or [[0, 0, 0, 0]]).reshape(-1)
+    out = np.zeros((1 << n) * 4, np.uint32)
+    lib().or_gen_eq_evals(_p(_arr(v)), _p(yy), n, _p(out))
+    return out.reshape(-1, 4)
+
+
+def fix_first_variable(evals, a):
+    """evals: M31 array (n,) or QM31 AoS array (n, 4); returns (n/2, 4)."""
+    ev = _arr(evals)
+    n = ev.shape[0]
+    out = np.zeros((n // 2) * 4, np.uint32)
+    f = lib().or_fix_first_variable_secure if ev.ndim == 2 else lib().or_fix_first_variable_base
+    f(_p(np.ascontiguousarray(ev).reshape(-1)), n, _p(_arr(a)), _p(out))
+    return out.reshape(-1, 4)
 
 
 # ---------------------------------------------------------------- planted mistakes

@@ -149,16 +149,21 @@ static inline uint64_t brev(uint64_t i, uint32_t bits) {
     return r;
 }
 static uint32_t log2_exact(uint64_t n) { uint32_t l = 0; while ((1ull << l) < n) l++; return l; }
+/* each pair (i, brev i) is swapped once, by its smaller index: pairs are disjoint */
 void or_bit_reverse_u32(uint32_t *a, uint64_t n) {
     uint32_t bits = log2_exact(n);
-    for (uint64_t i = 0; i < n; i++) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        uint64_t i = (uint64_t)ii;
         uint64_t j = brev(i, bits);
         if (j > i) { uint32_t t = a[i]; a[i] = a[j]; a[j] = t; }
     }
 }
 void or_bit_reverse_qm31(uint32_t *a, uint64_t n) {
     uint32_t bits = log2_exact(n);
-    for (uint64_t i = 0; i < n; i++) {
+#pragma omp parallel for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+        uint64_t i = (uint64_t)ii;
         uint64_t j = brev(i, bits);
         if (j > i) for (int k = 0; k < 4; k++) { uint32_t t = a[4 * i + k]; a[4 * i + k] = a[4 * j + k]; a[4 * j + k] = t; }
     }
@@ -345,27 +350,51 @@ void or_eval_at_point(const uint32_t *coeffs, uint32_t log_n, const uint32_t px[
 /* Batch inverse: Montgomery's trick (cuda/src/batch_inverse.cu:24-125).      */
 /* Inputs must be non-zero; the output is the exact inverse element-wise.    */
 /* ------------------------------------------------------------------------- */
+/* Independent chunks of OR_BINV_CHUNK elements run on separate threads: the
+   inverse of a non-zero element is unique, so the chunking does not change the
+   result. */
+#define OR_BINV_CHUNK 65536
 void or_batch_inverse_m31(const uint32_t *src, uint32_t *dst, uint64_t n) {
     if (!n) return;
-    uint32_t *pre = (uint32_t *)malloc(n * 4);
-    uint32_t acc = 1;
-    for (uint64_t i = 0; i < n; i++) { pre[i] = acc; acc = or_m31_mul(acc, src[i]); }
-    uint32_t inv = or_m31_inv(acc);
-    for (uint64_t i = n; i-- > 0;) { uint32_t s = src[i]; dst[i] = or_m31_mul(inv, pre[i]); inv = or_m31_mul(inv, s); }
-    free(pre);
+    int64_t n_chunks = (int64_t)((n + OR_BINV_CHUNK - 1) / OR_BINV_CHUNK);
+#pragma omp parallel
+    {
+        uint32_t *pre = (uint32_t *)malloc(OR_BINV_CHUNK * 4);
+#pragma omp for schedule(static)
+        for (int64_t ch = 0; ch < n_chunks; ch++) {
+            uint64_t lo = (uint64_t)ch * OR_BINV_CHUNK, hi = lo + OR_BINV_CHUNK < n ? lo + OR_BINV_CHUNK : n;
+            uint32_t acc = 1;
+            for (uint64_t i = lo; i < hi; i++) { pre[i - lo] = acc; acc = or_m31_mul(acc, src[i]); }
+            uint32_t inv = or_m31_inv(acc);
+            for (uint64_t i = hi; i-- > lo;) {
+                uint32_t s = src[i];
+                dst[i] = or_m31_mul(inv, pre[i - lo]);
+                inv = or_m31_mul(inv, s);
+            }
+        }
+        free(pre);
+    }
 }
 void or_batch_inverse_qm31(const uint32_t *src, uint32_t *dst, uint64_t n) {
     if (!n) return;
-    qm31 *pre = (qm31 *)malloc(n * sizeof(qm31));
-    qm31 acc = qm_from_m31(1);
-    for (uint64_t i = 0; i < n; i++) { pre[i] = acc; acc = qm_mul(acc, qm_load(src + 4 * i)); }
-    qm31 inv = qm_inv(acc);
-    for (uint64_t i = n; i-- > 0;) {
-        qm31 s = qm_load(src + 4 * i);
-        qm_store(qm_mul(inv, pre[i]), dst + 4 * i);
-        inv = qm_mul(inv, s);
+    int64_t n_chunks = (int64_t)((n + OR_BINV_CHUNK - 1) / OR_BINV_CHUNK);
+#pragma omp parallel
+    {
+        qm31 *pre = (qm31 *)malloc(OR_BINV_CHUNK * sizeof(qm31));
+#pragma omp for schedule(static)
+        for (int64_t ch = 0; ch < n_chunks; ch++) {
+            uint64_t lo = (uint64_t)ch * OR_BINV_CHUNK, hi = lo + OR_BINV_CHUNK < n ? lo + OR_BINV_CHUNK : n;
+            qm31 acc = qm_from_m31(1);
+            for (uint64_t i = lo; i < hi; i++) { pre[i - lo] = acc; acc = qm_mul(acc, qm_load(src + 4 * i)); }
+            qm31 inv = qm_inv(acc);
+            for (uint64_t i = hi; i-- > lo;) {
+                qm31 s = qm_load(src + 4 * i);
+                qm_store(qm_mul(inv, pre[i - lo]), dst + 4 * i);
+                inv = qm_mul(inv, s);
+            }
+        }
+        free(pre);
     }
-    free(pre);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -588,6 +617,41 @@ void or_wide_fib_constraint_quotients(uint64_t n_rows, uint32_t n_cols,
             num = qm_add(num, qm_scale(qm_load(coeff_powers + 4 * k), cst));
         }
         qm_store_soa(qm_scale(num, denom_inv[r]), out, (uint64_t)r);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Multilinear ops of the lookups (cuda/src/gkr.cu:4-38, cuda/src/mle.cu:4-30) */
+/* ------------------------------------------------------------------------- */
+/* evals[x] = v * prod_k (y_k if bit (n-1-k) of x is set else 1 - y_k): the
+   reference kernel walks the index from its low bit against y_{n-1} .. y_0
+   (gkr.cu:11-15) with factors (1 - y_k, y_k) (gkr.cu:25-28). */
+void or_gen_eq_evals(const uint32_t v[4], const uint32_t *y, uint32_t n, uint32_t *evals) {
+    for (uint64_t x = 0; x < (1ull << n); x++) {
+        qm31 e = qm_load(v);
+        uint64_t t = x;
+        for (int k = (int)n - 1; k >= 0; k--) {
+            qm31 yk = qm_load(y + 4 * k);
+            e = qm_mul(e, (t & 1) ? yk : qm_sub(qm_from_m31(1), yk));
+            t >>= 1;
+        }
+        qm_store(e, evals + 4 * x);
+    }
+}
+/* out[i] = lhs + (rhs - lhs) * a, lhs = evals[i], rhs = evals[i + n/2] (mle.cu:4-12);
+   M31 evaluations in, QM31 (AoS) out */
+void or_fix_first_variable_base(const uint32_t *evals, uint64_t n, const uint32_t a[4], uint32_t *out) {
+    for (uint64_t i = 0; i < n / 2; i++) {
+        uint32_t l = evals[i], r = evals[i + n / 2];
+        qm31 t = qm_scale(qm_load(a), or_m31_sub(r, l));
+        t.x.a = or_m31_add(t.x.a, l);
+        qm_store(t, out + 4 * i);
+    }
+}
+void or_fix_first_variable_secure(const uint32_t *evals, uint64_t n, const uint32_t a[4], uint32_t *out) {
+    for (uint64_t i = 0; i < n / 2; i++) {
+        qm31 l = qm_load(evals + 4 * i), r = qm_load(evals + 4 * (i + n / 2));
+        qm_store(qm_add(l, qm_mul(qm_sub(r, l), qm_load(a))), out + 4 * i);
     }
 }
 

@@ -115,6 +115,12 @@ void or_wide_fib_constraint_quotients(uint64_t n_rows, uint32_t n_cols,
                                       const uint32_t *coeff_powers,
                                       const uint32_t *denom_inv, uint32_t *const out[4]);
 
+/* ---- multilinear ops of the lookups (cuda/src/gkr.cu:4-38, cuda/src/mle.cu:4-30) ----
+ * QM31 values AoS (4 words); variable 0 is the most significant index bit. */
+void or_gen_eq_evals(const uint32_t v[4], const uint32_t *y, uint32_t n_vars, uint32_t *evals);
+void or_fix_first_variable_base(const uint32_t *evals, uint64_t n, const uint32_t a[4], uint32_t *out);
+void or_fix_first_variable_secure(const uint32_t *evals, uint64_t n, const uint32_t a[4], uint32_t *out);
+
 /* ---- planted mistakes, for the oracle's pinning tests only ----
  * or_set_mutation(m) makes later calls compute a deliberately wrong variant;
  * 0 (OR_MUT_NONE) restores the restatement.  Nothing but tests/test_oracle.py

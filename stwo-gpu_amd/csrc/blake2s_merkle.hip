@@ -352,6 +352,7 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   std::vector<std::vector<const uint32_t*>> by_layer(max_log + 1);
   for (uint32_t i = 0; i < n_cols; i++) {
     require(logs[i] <= max_log, "merkle: column larger than the tree");
+    require(logs[i] <= 30, "merkle: columns hold at most 2^30 values");
     by_layer[logs[i]].push_back(cols[i]);
   }
   std::vector<const uint32_t*> flat;
@@ -474,7 +475,8 @@ using sh::merkle::Hash;
 extern "C" void commit_on_first_layer(size_t size, size_t amount_of_columns, uint32_t* const* columns,
                                       StwoBlake2sHash* result) {
   guarded([&] {
-    require(size <= 0xffffffffull, "commit_on_first_layer: size too large");
+    // column offsets are formed in 32 bits (gmem.hpp): injected columns hold at most 2^30 values
+    require(size <= (amount_of_columns ? (1ull << 30) : 0xffffffffull), "commit_on_first_layer: size too large");
     if (!size) return;
     timed("merkle_leaf", [&] {
       merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(
@@ -487,7 +489,8 @@ extern "C" void commit_on_first_layer(size_t size, size_t amount_of_columns, uin
 extern "C" void commit_on_layer_with_previous(size_t size, size_t amount_of_columns, uint32_t* const* columns,
                                               const StwoBlake2sHash* previous_layer, StwoBlake2sHash* result) {
   guarded([&] {
-    require(size <= 0xffffffffull, "commit_on_layer_with_previous: size too large");
+    require(size <= (amount_of_columns ? (1ull << 30) : 0xffffffffull),
+            "commit_on_layer_with_previous: size too large");
     if (!size) return;
     timed("merkle_node", [&] {
       merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(

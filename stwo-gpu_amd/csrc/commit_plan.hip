@@ -242,7 +242,28 @@ extern "C" int stwo_hip_commit_plan_create(StwoCommitPlan** out, uint32_t n_cols
     for (uint32_t i = 0; i < n_cols; i++) lde_logs[i] = log_sizes[i] + log_blowup;
     p.merkle = std::make_unique<merkle::MerklePlan>(n_cols, (const uint32_t* const*)ldes, lde_logs.data(),
                                                     (merkle::Hash* const*)layers, max_log);
-    if (autotune) p.autotune();
+    if (autotune) {
+      // the autotuner runs the plan several times on the caller's buffers; a column whose
+      // coefficients alias its evaluations is interpolated in place by every run, so its
+      // evaluations are kept aside and put back before create returns
+      struct Saved {
+        void* col;
+        size_t bytes;
+        DevBuf copy;
+      };
+      std::vector<Saved> saved;
+      for (uint32_t i = 0; i < n_cols; i++) {
+        if ((const void*)coeffs[i] != (const void*)evals[i]) continue;
+        const size_t bytes = sizeof(uint32_t) << log_sizes[i];
+        Saved sv{coeffs[i], bytes, DevBuf(bytes)};
+        hip_check(hipMemcpyAsync(sv.copy.p, sv.col, bytes, hipMemcpyDeviceToDevice, ctx().cur), "hipMemcpyAsync D2D");
+        saved.push_back(std::move(sv));
+      }
+      p.autotune();
+      for (Saved& sv : saved)
+        hip_check(hipMemcpyAsync(sv.col, sv.copy.p, sv.bytes, hipMemcpyDeviceToDevice, ctx().cur),
+                  "hipMemcpyAsync D2D");
+    }
     sync();
     *out = holder.release();
   });

@@ -227,7 +227,8 @@ size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L) {
 // d_out[c] = value at d_point (x, y) of the polynomial with coefficients d_cols[c] (2^L each).
 void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
                                  qm31* d_out, void* d_scratch) {
-  require(L >= 1 && L <= 31 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
+  // coefficient offsets are formed in 32 bits (gmem.hpp): at most 2^30 coefficients
+  require(L >= 1 && L <= 30 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
   eap::Factors* F = (eap::Factors*)d_scratch;
   if (L >= eap::LO_BITS) {
     const uint32_t n_hi = 1u << (L - eap::LO_BITS);
@@ -268,7 +269,7 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
 }
 
 qm31 eval_at_point_sync(const uint32_t* coeffs, uint64_t n, qm31 px, qm31 py) {
-  require(is_pow2(n) && n <= (1ull << 31), "eval_at_point: size must be a power of two");
+  require(is_pow2(n) && n <= (1ull << 30), "eval_at_point: size must be a power of two <= 2^30");
   uint32_t L = log2_exact(n);
   if (L == 0) {
     uint32_t c0;

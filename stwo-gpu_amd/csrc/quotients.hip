@@ -251,6 +251,17 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* con
   if (threadIdx.x == 0) bs[0] = Batch{0, count, c, sa[0], sb[0], qpow(rc, count), px.x, py.x, px.y, py.y};
 }
 
+// the quotient of an empty sample set: all four coordinates zero
+__global__ __launch_bounds__(256) void zero_kernel(uint32_t n, uint32_t* o0, uint32_t* o1, uint32_t* o2,
+                                                   uint32_t* o3) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  o0[i] = 0;
+  o1[i] = 0;
+  o2[i] = 0;
+  o3[i] = 0;
+}
+
 }  // namespace quot
 
 static uint32_t padded_slots(uint32_t n_cols) { return (n_cols + quot::QG - 1) / quot::QG * quot::QG; }
@@ -304,7 +315,7 @@ void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t 
                                              const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
                                              const qm31* d_values, const qm31* d_rc, void* d_scratch,
                                              uint32_t* const out[4]) {
-  require(log_size >= 1 && log_size <= 31 && n_cols >= 1, "accumulate_quotients: bad sizes");
+  require(log_size >= 1 && log_size <= 30 && n_cols >= 1, "accumulate_quotients: bad sizes");
   const uint32_t n_slots = padded_slots(n_cols);
   quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
   quot::Batch* bs = (quot::Batch*)((char*)d_scratch + slot_bytes(n_slots));
@@ -381,8 +392,21 @@ void accumulate_quotients_async(uint32_t hi, uint32_t hs, uint32_t domain_size, 
                                 qm31 rc, uint32_t n_batches, const StwoCirclePointQM31* points,
                                 const uint32_t* sizes, const uint32_t* col_idx, const StwoQM31* values,
                                 uint32_t* const out[4]) {
-  require(is_pow2(domain_size) && domain_size >= 2, "accumulate_quotients: domain size must be a power of two >= 2");
+  // column offsets are formed in 32 bits (gmem.hpp): at most 2^30 rows
+  require(is_pow2(domain_size) && domain_size >= 2 && domain_size <= (1u << 30),
+          "accumulate_quotients: domain size must be a power of two in 2 .. 2^30");
   uint32_t log_size = log2_exact(domain_size);
+  uint64_t n_samples = 0;
+  for (uint32_t b = 0; b < n_batches; b++) n_samples += sizes[b];
+  if (n_samples == 0) {
+    // every numerator is zero, so is the accumulation: no column is read
+    timed("quotients_zero", [&] {
+      quot::zero_kernel<<<ceil_div(domain_size, 256), 256, 0, ctx().cur>>>(domain_size, out[0], out[1], out[2],
+                                                                             out[3]);
+    });
+    return;
+  }
+  require(n_cols >= 1 && h_cols != nullptr, "accumulate_quotients: samples name columns but none are given");
   std::vector<quot::Batch> bs(n_batches);
   std::vector<quot::Sample> ss;
   uint32_t off = 0;

@@ -96,6 +96,10 @@ extern "C" int stwo_hip_twiddles(uint32_t initial_index, uint32_t step_size, uin
 extern "C" int stwo_hip_release_twiddles(void) {
   return guarded([] {
     sync();
+    // proof-server jobs on the slot streams read the cached tables too: wait for every
+    // stream, so that no block handed back to the pool here is still being read (new jobs
+    // are enqueued under the library lock and fetch the tables again)
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     for (auto& kv : ctx().twiddles) {
       ctx().pool.free(kv.second.tw);
       ctx().pool.free(kv.second.itw);

@@ -93,6 +93,9 @@ def _declare(L):
         "accumulate_quotients": (None, [u32, u32, u32, vp, sz, QM31, vp, u32p, u32, C.POINTER(QM31), u32p,
                                         u32, vp, vp, vp, vp, u32]),
         "evaluate_wide_fibonacci_constraint_quotients_on_domain": (None, [vp, vp, vp, vp, pp, vp, vp, u32, u32]),
+        "gen_eq_evals": (None, [QM31, C.POINTER(QM31), u32, vp, u32]),
+        "fix_first_variable_base_field": (None, [vp, sz, QM31, vp]),
+        "fix_first_variable_secure_field": (None, [vp, sz, QM31, vp]),
         # extensions
         "stwo_hip_init": (C.c_int, [C.c_int]),
         "stwo_hip_last_status": (C.c_int, []),
@@ -518,6 +521,26 @@ def wide_fib_constraint_quotients(out: SecureColumn, trace, powers: SecureFieldV
         *[c.ptr for c in out.columns], _ptr_array([c.ptr for c in trace]), powers.ptr, denom_inv.ptr,
         len(out), len(trace))
     check()
+
+
+def gen_eq_evals(y, v) -> SecureFieldVec:
+    """GkrOps::gen_eq_evals (lookups/gkr.rs:12-27): v * eq(x, y) for every x in
+    {0,1}^len(y), variable 0 the most significant index bit."""
+    out = SecureFieldVec.uninitialized(1 << len(y))
+    ys = (QM31 * max(1, len(y)))(*[qm31(t) for t in y])
+    lib().gen_eq_evals(qm31(v), ys, len(y), out.ptr, out.size)
+    check()
+    return out
+
+
+def fix_first_variable(mle, assignment) -> SecureFieldVec:
+    """MleOps<BaseField | SecureField>::fix_first_variable (lookups/mle.rs:12-57)."""
+    out = SecureFieldVec.uninitialized(mle.size // 2)
+    f = (lib().fix_first_variable_secure_field if isinstance(mle, SecureFieldVec)
+         else lib().fix_first_variable_base_field)
+    f(mle.ptr, mle.size, qm31(assignment), out.ptr)
+    check()
+    return out
 
 
 # ----------------------------------------------------------------- extensions

@@ -200,3 +200,27 @@ def conj_line(point, value, y):
     c = qm_sub(qm_conj(point[1]), point[1])
     slope = qm_div(qm_sub(qm_conj(value), value), c)
     return qm_add(value, qm_mul(slope, qm_sub(y, point[1])))
+
+
+# ---------------------------------------------------------------- multilinear extensions
+def eq_poly(x_bits, r):
+    """eq(x, r) = prod_k (x_k r_k + (1 - x_k)(1 - r_k)) for boolean x."""
+    e = qm(1)
+    for xb, rk in zip(x_bits, r):
+        e = qm_mul(e, rk if xb else qm_sub(qm(1), rk))
+    return e
+
+
+def index_bits(x, n):
+    """Variable k of index x is bit (n-1-k): variable 0 is the most significant bit."""
+    return [(x >> (n - 1 - k)) & 1 for k in range(n)]
+
+
+def mle_eval(evals, r):
+    """Multilinear extension of evals (length 2^n) at r: sum_x f(x) eq(x, r)."""
+    n = len(r)
+    total = qm(0)
+    for x, f in enumerate(evals):
+        fx = f if isinstance(f, tuple) else qm(int(f))
+        total = qm_add(total, qm_mul(fx, eq_poly(index_bits(x, n), r)))
+    return total

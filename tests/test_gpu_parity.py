@@ -584,3 +584,136 @@ def test_gather_rows_and_auth_paths():
             assert (paths[qi, j] == layers[max_log - j][(int(leaf) >> j) ^ 1]).all()
     with pytest.raises(S.StwoHipError):
         S.merkle_auth_paths(tree.layers, np.array([1 << max_log], np.uint32))
+
+
+# ------------------------------------------------------------------ lookups/gkr.rs, lookups/mle.rs
+GKR_TEST_Y = [(7, 3, 5, 6), (1, 1, 9, 7), (3, 5, 6, 1), (1, 9, 7, 3), (5, 6, 1, 1), (9, 7, 3, 5), (6, 1, 1, 9)]
+
+
+def test_gen_eq_evals_reference_input():
+    # gkr.rs:46-63: y from [7, 3, 5, 6, 1, 1, 9] repeated 4 times in chunks of 4, v = 2
+    got = S.gen_eq_evals(GKR_TEST_Y, (2, 0, 0, 0)).to_numpy()
+    assert (got == O.gen_eq_evals(GKR_TEST_Y, (2, 0, 0, 0))).all()
+
+
+@pytest.mark.parametrize("n_vars", [0, 1, 9, 10, 11, 20])
+def test_gen_eq_evals_sizes(n_vars):
+    rng = random.Random(600 + n_vars)
+    y = [rand_qm31(rng) for _ in range(n_vars)]
+    v = rand_qm31(rng)
+    got = S.gen_eq_evals(y, v).to_numpy()
+    assert (got == O.gen_eq_evals(y, v)).all()
+
+
+@pytest.mark.parametrize("secure", [False, True])
+def test_fix_first_variable_reference_input(secure):
+    # mle.rs:70-109: 2^8 evaluations 0..255 (QM31: words 0..1023 in chunks of 4), assignment (7, 12, 3, 2)
+    a = (7, 12, 3, 2)
+    if secure:
+        ev = np.arange(4 << 8, dtype=np.uint32).reshape(-1, 4)
+        got = S.fix_first_variable(S.SecureFieldVec.from_numpy(ev), a).to_numpy()
+    else:
+        ev = np.arange(1 << 8, dtype=np.uint32)
+        got = S.fix_first_variable(S.BaseFieldVec.from_numpy(ev), a).to_numpy()
+    assert (got == O.fix_first_variable(ev, a)).all()
+
+
+@pytest.mark.parametrize("secure", [False, True])
+def test_fix_first_variable_large(secure):
+    rng = random.Random(700)
+    a = rand_qm31(rng)
+    n = 1 << 20
+    if secure:
+        ev = rand_m31(4 * n, 701).reshape(-1, 4)
+        got = S.fix_first_variable(S.SecureFieldVec.from_numpy(ev), a).to_numpy()
+    else:
+        ev = rand_m31(n, 702)
+        got = S.fix_first_variable(S.BaseFieldVec.from_numpy(ev), a).to_numpy()
+    assert (got == O.fix_first_variable(ev, a)).all()
+
+
+# ------------------------------------------------------------------ size guards (checked on the host, before any launch)
+def test_oversize_calls_fail_before_launch():
+    L = S.lib()
+    col = S.BaseFieldVec.from_numpy(np.arange(1 << 10, dtype=np.uint32))
+    out = S.SecureColumn.uninitialized(1 << 10)
+    samples = S.QuotientSamples([(S.random_circle_point_qm31(np.random.default_rng(1)), [(0, (1, 2, 3, 4))])])
+    dom31 = S.canonic_half_coset(31)
+    st = L.stwo_hip_accumulate_quotients_async(dom31.initial_index, dom31.step_size, 1 << 31,
+                                               S._ptr_array([col.ptr]), 1, S.qm31((1, 0, 0, 0)), samples.n,
+                                               S.C.cast(samples.points, S.vp), samples.sizes.ctypes.data_as(S.u32p),
+                                               samples.idx.ctypes.data_as(S.u32p), samples.vals, out.ptrs())
+    assert st == 1 and "2^30" in S.last_error()
+    L.stwo_hip_clear_error()
+    vals = (S.QM31 * 1)()
+    st = L.stwo_hip_eval_at_point_columns(S._ptr_array([col.ptr]), 1, 31, S.qm31((1, 0, 0, 0)), S.qm31((0, 1, 0, 0)),
+                                          vals)
+    assert st == 1
+    L.stwo_hip_clear_error()
+    L.eval_at_point(col.ptr, 1 << 31, S.qm31((1, 0, 0, 0)), S.qm31((0, 1, 0, 0)))
+    assert L.stwo_hip_last_status() == 1
+    L.stwo_hip_clear_error()
+    table = L.copy_device_pointer_vec_from_host_to_device(S._ptr_array([col.ptr]), 1)
+    hashes = S.Blake2sHashVec.uninitialized(1)
+    L.commit_on_first_layer(1 << 31, 1, table, hashes.ptr)
+    assert L.stwo_hip_last_status() == 1
+    L.stwo_hip_clear_error()
+    L.cuda_free_memory(table)
+    # the library still works
+    S.bit_reverse_column(col)
+    assert (col.to_numpy() == O.bit_reverse(np.arange(1 << 10, dtype=np.uint32))).all()
+
+
+def test_quotients_without_samples_are_zero():
+    # no sampled column: every numerator is zero and no column is read (none is given)
+    L = S.lib()
+    m = 10
+    dom = S.canonic_half_coset(m)
+    out = S.SecureColumn.from_numpy(np.full((4, 1 << m), 5, np.uint32))
+    pts = (S.PointQM31 * 1)(S.PointQM31(S.qm31((1, 2, 3, 4)), S.qm31((5, 6, 7, 8))))
+    sizes = np.zeros(1, np.uint32)
+    idx = np.zeros(1, np.uint32)
+    st = L.stwo_hip_accumulate_quotients_async(dom.initial_index, dom.step_size, 1 << m, None, 0,
+                                               S.qm31((3, 0, 0, 0)), 1, S.C.cast(pts, S.vp),
+                                               sizes.ctypes.data_as(S.u32p), idx.ctypes.data_as(S.u32p),
+                                               (S.QM31 * 1)(), out.ptrs())
+    assert st == 0
+    S.synchronize()
+    assert not out.to_numpy().any()
+    assert (O.accumulate_quotients(dom.initial_index, dom.step_size, m, [], (3, 0, 0, 0),
+                                   [(((1, 2, 3, 4), (5, 6, 7, 8)), [])]) == 0).all()
+    # a sample naming a column when none is given is rejected
+    sizes[0] = 1
+    st = L.stwo_hip_accumulate_quotients_async(dom.initial_index, dom.step_size, 1 << m, None, 0,
+                                               S.qm31((3, 0, 0, 0)), 1, S.C.cast(pts, S.vp),
+                                               sizes.ctypes.data_as(S.u32p), idx.ctypes.data_as(S.u32p),
+                                               (S.QM31 * 1)(), out.ptrs())
+    assert st == 1
+    L.stwo_hip_clear_error()
+
+
+def test_commit_plan_autotune_with_aliased_coefficients():
+    # coeffs[i] == evals[i]: the autotuner's runs must leave the evaluations in place, so the
+    # first run commits the LDE of the caller's evaluations
+    logs = [14, 15, 15, 12, 9]
+    blow = 1
+    m = max(logs) + blow
+    root = S.canonic_half_coset(m)
+    tw, itw = S.cached_twiddles(root)
+    evals_np = [rand_m31(1 << l, 8800 + i) for i, l in enumerate(logs)]
+    evals = [S.BaseFieldVec.from_numpy(e) for e in evals_np]
+    ldes = [S.BaseFieldVec.uninitialized(1 << (l + blow)) for l in logs]
+    plan = S.CommitPlan(evals, evals, ldes, logs, blow, tw, itw, 1 << root.log_size, autotune=True)
+    for i in range(len(logs)):
+        assert (evals[i].to_numpy() == evals_np[i]).all(), f"evaluations of column {i} kept"
+    plan.run(True)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    want_lde = []
+    for i, (e, l) in enumerate(zip(evals_np, logs)):
+        oc, ol = O.lde_columns([e], blow, otw, oitw)
+        assert (evals[i].to_numpy() == oc[0]).all(), f"coefficients of column {i}"
+        assert (ldes[i].to_numpy() == ol[0]).all(), f"LDE of column {i}"
+        want_lde.append(ol[0])
+    assert (plan.root() == O.merkle_commit(want_lde)[0]).all()
+    plan.close()

@@ -584,6 +584,64 @@ def test_wide_fib_check_rejects_planted_power_order():
     assert not _wf_matches(out, trace, dinv, alpha)
 
 
+# ------------------------------------------------------------------ multilinear ops
+# gen_eq_evals and fix_first_variable (gkr.cu, mle.cu) against the multilinear
+# extension written from its definition, sum_x f(x) eq(x, r), with variable 0
+# the most significant index bit.
+GKR_TEST_Y = [(7, 3, 5, 6), (1, 1, 9, 7), (3, 5, 6, 1), (1, 9, 7, 3), (5, 6, 1, 1), (9, 7, 3, 5),
+              (6, 1, 1, 9)]  # lookups/gkr.rs:52-58: [7, 3, 5, 6, 1, 1, 9] repeated 4 times, in chunks of 4
+
+
+def test_gen_eq_evals_definition():
+    two = R.qm(2)
+    out = O.gen_eq_evals(GKR_TEST_Y, two)
+    n = len(GKR_TEST_Y)
+    for x in range(1 << n):
+        assert tuple(int(v) for v in out[x]) == R.qm_mul(two, R.eq_poly(R.index_bits(x, n), GKR_TEST_Y))
+    # a boolean y makes eq the indicator of the index whose bits (MSB first) are y
+    yb = [R.qm(b) for b in (1, 0, 1, 1)]
+    ind = O.gen_eq_evals(yb, R.qm(1))
+    assert [int(v[0]) for v in ind] == [int(x == 0b1011) for x in range(16)] and not ind[:, 1:].any()
+    assert O.gen_eq_evals([], (5, 6, 7, 8)).tolist() == [[5, 6, 7, 8]]
+
+
+@pytest.mark.parametrize("secure", [False, True])
+def test_fix_first_variable_chain_is_the_multilinear_extension(secure):
+    rng = random.Random(31)
+    n = 5
+    if secure:
+        f = np.array([[rng.randrange(P) for _ in range(4)] for _ in range(1 << n)], np.uint32)
+        fv = [tuple(int(v) for v in row) for row in f]
+    else:
+        f = np.array([rng.randrange(P) for _ in range(1 << n)], np.uint32)
+        fv = [int(v) for v in f]
+    r = [tuple(rng.randrange(P) for _ in range(4)) for _ in range(n)]
+    cur = O.fix_first_variable(f, r[0])
+    for k in range(1, n):
+        cur = O.fix_first_variable(cur, r[k])
+    assert tuple(int(v) for v in cur[0]) == R.mle_eval(fv, r)
+    # assignments 0 and 1 select the low and high halves
+    lo, hi = O.fix_first_variable(f, R.qm(0)), O.fix_first_variable(f, R.qm(1))
+    half = 1 << (n - 1)
+    if secure:
+        assert (lo == f[:half]).all() and (hi == f[half:]).all()
+    else:
+        assert (lo[:, 0] == f[:half]).all() and (hi[:, 0] == f[half:]).all() and not lo[:, 1:].any()
+
+
+def test_gen_eq_evals_weights_the_multilinear_extension():
+    # sum_x gen_eq_evals(r, 1)[x] f(x) = MLE(f)(r)
+    rng = random.Random(32)
+    n = 4
+    f = [rng.randrange(P) for _ in range(1 << n)]
+    r = [tuple(rng.randrange(P) for _ in range(4)) for _ in range(n)]
+    w = O.gen_eq_evals(r, R.qm(1))
+    tot = R.qm(0)
+    for x in range(1 << n):
+        tot = R.qm_add(tot, R.qm_scale(tuple(int(v) for v in w[x]), f[x]))
+    assert tot == R.mle_eval(f, r)
+
+
 # ------------------------------------------------------------------ inputs
 def test_uniform_generator():
     a = O.fill_uniform_m31(1 << 16, 0x5EED0002)
