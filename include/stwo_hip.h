@@ -320,6 +320,16 @@ int stwo_hip_eval_at_point_columns(const uint32_t *const *columns, uint32_t n_co
  * OODS evaluation over (column, point) pairs; points: HOST). */
 int stwo_hip_eval_at_points_columns(const uint32_t *const *columns, uint32_t n_cols, uint32_t log_size,
                                     const StwoCirclePointQM31 *points, uint32_t n_points, StwoQM31 *values);
+/* No counterpart in the reference: stwo_hip_eval_at_points_columns for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; no hipMalloc/hipFree or pool allocation: the pointer
+ * table, points, values and partial sums live in a fixed 32 MiB device array (one per device,
+ * held until the process ends; e.g. 512 columns up to log 23 at one point).  Calls needing
+ * more return STWO_HIP_ERR_ARG before any copy or launch.  Calls that overlap on one device
+ * share the array and race.  Synchronous, like the allocating form. */
+int stwo_hip_eval_at_points_columns_noalloc(const uint32_t *const *columns, uint32_t n_cols, uint32_t log_size,
+                                            const StwoCirclePointQM31 *points, uint32_t n_points,
+                                            StwoQM31 *values);
 
 /* Proof of work (GrindOps<Blake2sChannel>::grind, which the reference delegates to the CPU
  * SimdBackend, stwo_gpu_backend/src/backend.rs:14-18): the smallest nonce whose

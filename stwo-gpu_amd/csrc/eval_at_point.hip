@@ -350,6 +350,70 @@ extern "C" int stwo_hip_eval_at_points_columns(const uint32_t* const* columns, u
   });
 }
 
+// ---- the batched evaluation with no device allocation ----------------------------------
+// For callers that cannot allocate (code under graph capture, serving loops, harnesses that
+// forbid hipMalloc): the pointer table, the points, the values and the partial sums live in
+// this fixed array instead of pooled buffers.  Each device holds its own copy until the
+// process ends.  Every call on a device shares that copy, so calls that overlap on one device
+// race; nothing in it has to start at zero (every partial is written before it is read).
+namespace {
+constexpr size_t NOALLOC_BYTES = 32u << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+constexpr int MAX_DEVICES = 64;
+void* g_scratch[MAX_DEVICES];  // device address of `scratch` per device, under the library lock
+
+// The device address of this device's `scratch` (the array's name is only a host-side
+// stand-in); looked up on the first call on each device, with that device current.
+void* noalloc_scratch() {
+  const int dev = ctx().device;
+  require(dev >= 0 && dev < MAX_DEVICES, "eval_at_points_columns_noalloc: device index out of range");
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    require(p != nullptr, "eval_at_points_columns_noalloc: no device address for the scratch array");
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" int stwo_hip_eval_at_points_columns_noalloc(const uint32_t* const* columns, uint32_t n_cols,
+                                                       uint32_t log_size, const StwoCirclePointQM31* points,
+                                                       uint32_t n_points, StwoQM31* values) {
+  return guarded([&] {
+    require(columns != nullptr && values != nullptr && points != nullptr && n_cols >= 1 && n_points >= 1 &&
+                log_size >= 1 && log_size <= 30 && n_cols <= 65535,
+            "eval_at_points_columns_noalloc: bad arguments");
+    const size_t t_bytes = align256((size_t)n_cols * sizeof(void*));
+    const size_t p_bytes = align256(2 * sizeof(qm31) * (size_t)n_points);
+    const size_t o_bytes = align256(sizeof(qm31) * (size_t)n_cols * n_points);
+    const size_t need = t_bytes + p_bytes + o_bytes + eval_at_point_columns_scratch_bytes(n_cols, log_size);
+    if (need > NOALLOC_BYTES)
+      fail(STWO_HIP_ERR_ARG,
+           "eval_at_points_columns_noalloc: %u columns of log %u at %u points need %zu scratch bytes; the "
+           "fixed scratch holds %zu",
+           n_cols, log_size, n_points, need, NOALLOC_BYTES);
+    char* base = (char*)noalloc_scratch();
+    const uint32_t** table = (const uint32_t**)base;
+    qm31* dpt = (qm31*)(base + t_bytes);
+    qm31* out = (qm31*)(base + t_bytes + p_bytes);
+    void* work = base + t_bytes + p_bytes + o_bytes;
+    // points in the kernels' (x, y) layout; QM31 words are (a, b, c, d) in both
+    static_assert(sizeof(StwoCirclePointQM31) == 2 * sizeof(qm31), "point layout");
+    hipStream_t s = ctx().cur;
+    hip_check(hipMemcpyAsync(table, columns, (size_t)n_cols * sizeof(void*), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync H2D");
+    hip_check(hipMemcpyAsync(dpt, points, 2 * sizeof(qm31) * (size_t)n_points, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync H2D");
+    for (uint32_t k = 0; k < n_points; k++)
+      eval_at_point_columns_async(table, n_cols, log_size, dpt + 2 * k, out + (size_t)k * n_cols, work);
+    hip_check(hipMemcpyAsync(values, out, sizeof(qm31) * n_cols * (size_t)n_points, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync D2H");
+    sync();
+  });
+}
+
 extern "C" int stwo_hip_eval_at_point_columns(const uint32_t* const* columns, uint32_t n_cols, uint32_t log_size,
                                               StwoQM31 point_x, StwoQM31 point_y, StwoQM31* values) {
   const StwoCirclePointQM31 p{point_x, point_y};

@@ -137,6 +137,7 @@ def _declare(L):
         "stwo_hip_server_limit_work": (C.c_int, [vp, u64]),
         "stwo_hip_gather_rows": (C.c_int, [vp, u32, u64, u32p, u32, u32p]),
         "stwo_hip_eval_at_points_columns": (C.c_int, [vp, u32, u32, vp, u32, vp]),
+        "stwo_hip_eval_at_points_columns_noalloc": (C.c_int, [vp, u32, u32, vp, u32, vp]),
         "stwo_hip_merkle_auth_paths": (C.c_int, [vp, u32, u32p, u32, u32p]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
         "stwo_hip_server_wait": (C.c_int, [vp, u32, C.POINTER(u64), C.POINTER(C.c_int), u32p, u64,
@@ -769,13 +770,14 @@ def merkle_auth_paths(layers, leaves):
     return out
 
 
-def eval_at_points_columns(cols, points):
-    """[[column c at points[k] for c] for k]: `points` is a list of QM31 circle points (px, py)."""
+def eval_at_points_columns(cols, points, noalloc=False):
+    """[[column c at points[k] for c] for k]: `points` is a list of QM31 circle points (px, py).
+    noalloc: the entry point that allocates no device memory (fixed per-device scratch)."""
     log = cols[0].size.bit_length() - 1
     pts = (PointQM31 * len(points))(*[PointQM31(qm31(px), qm31(py)) for px, py in points])
     out = (QM31 * (len(cols) * len(points)))()
-    _c(lib().stwo_hip_eval_at_points_columns(_ptr_array([c.ptr for c in cols]), len(cols), log, C.cast(pts, vp),
-                                             len(points), C.cast(out, vp)))
+    fn = lib().stwo_hip_eval_at_points_columns_noalloc if noalloc else lib().stwo_hip_eval_at_points_columns
+    _c(fn(_ptr_array([c.ptr for c in cols]), len(cols), log, C.cast(pts, vp), len(points), C.cast(out, vp)))
     return [[from_qm31(out[k * len(cols) + c]) for c in range(len(cols))] for k in range(len(points))]
 
 
