@@ -17,7 +17,12 @@
 //   * Within a pass the layers are done radix-8 in registers; between radix
 //     stages the tile is exchanged through LDS.  The next stage's twiddles are
 //     loaded into registers before each exchange, so their latency overlaps the
-//     barriers.
+//     barriers.  Where a wave's 64 lanes differ only in index bits below the stage's
+//     twiddle bits (strided tiles with >= 64-word runs, and the upper stages of the
+//     contiguous passes) the twiddles are wave-uniform and come through the scalar
+//     cache into SGPRs: no vector-memory instructions or VGPRs for them, and their
+//     doubling runs on the scalar ALU (256 columns of log 20: fused LDE pass 1.35 ->
+//     1.20 ms).
 //   * Butterflies multiply by twice the twiddle (mul_dbl in field.hpp), one
 //     instruction fewer per product mod P than a plain multiplication; the passes
 //     are VALU-bound (about 12 instructions per butterfly).
@@ -182,6 +187,11 @@ struct Pass {
   template <int LO, int W>
   __device__ __forceinline__ void load_tw(uint32_t (&tw)[NTW]) const {
     constexpr int R = 1 << W;
+    // the 64 lanes of a wave differ in the low 6 bits of threadIdx.x; when those land below the
+    // twiddle's index bits (T + LO >= 6) every lane of the wave needs the same twiddles, so they
+    // are loaded once per wave through the scalar cache into SGPRs
+    constexpr bool UNIFORM = T + LO >= 6;
+    const uint32_t tid = UNIFORM ? (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x & ~63u) : threadIdx.x;
     static_for<0, W>([&](auto si) {
       constexpr int BL = si.value;
       constexpr int BIT = LO + BL;
@@ -191,10 +201,15 @@ struct Pass {
       const SH_GLOBAL uint32_t* lvl = level(BIT);
 #pragma unroll
       for (int gi = 0; gi < E / R; gi++) {
-        const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
+        const uint32_t pb = spread<T + LO, W>(tid + NT * gi);
 #pragma unroll
-        for (int rh = 0; rh < (R >> (BL + 1)); rh++)
-          tw[tw_slot<W, BL>(gi, rh)] = at(lvl, tw_index<BIT>(pb | ((uint32_t)(rh << (BL + 1)) << (T + LO))));
+        for (int rh = 0; rh < (R >> (BL + 1)); rh++) {
+          const uint32_t i = tw_index<BIT>(pb | ((uint32_t)(rh << (BL + 1)) << (T + LO)));
+          if constexpr (UNIFORM)
+            tw[tw_slot<W, BL>(gi, rh)] = ((const __attribute__((address_space(4))) uint32_t*)lvl)[i];
+          else
+            tw[tw_slot<W, BL>(gi, rh)] = at(lvl, i);
+        }
       }
     });
   }
