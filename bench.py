@@ -339,7 +339,9 @@ def bench_batch_inverse(args):
 
 
 def bench_fri(args):
-    """fold_circle_into_line of a QM31 circle evaluation at log 22, then fold_line down to log 4."""
+    """fold_circle_into_line of a QM31 circle evaluation at log 22, then fold_line down to log 4,
+    every folding coefficient seeded up front: the fused chain (each block folds a 2^13-value
+    range 13 times in LDS; two launches), and beside it the per-layer path (18 launches)."""
     import numpy as np
     import stwo_hip as S
 
@@ -355,15 +357,27 @@ def bench_fri(args):
     alphas = [S.random_qm31(rng) for _ in range(len(layers))]
 
     def step():
+        S.fri_fold_chain_async(src, layers, alphas, itw, tree_len)
+
+    def step_per_layer():
         S.fold_circle_into_line_async(layers[0], src, alphas[0], itw, tree_len - len(layers[0]))
         for i in range(1, len(layers)):
             S.fold_line_async(layers[i - 1], layers[i], alphas[i], itw, tree_len - len(layers[i - 1]))
     ms = _timed_steps(S, step, args.steps, args.warmup)
-    ks = _kernel_avg(S, step, ["fold_circle_into_line", "fold_line"])
+    ms_layers = _timed_steps(S, step_per_layer, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["fri_fold_chain", "fri_fold_chain_tail"])
+    ks.update(_kernel_avg(S, step_per_layer, ["fold_circle_into_line", "fold_line"]))
+    # algorithmic bytes: every layer read once and written once (the circle fold also reads the
+    # layer it accumulates into), plus the twiddle words each fold reads
     circle_bytes = 16 * (1 << L) + 2 * 16 * (1 << (L - 1)) + 4 * (1 << (L - 2))
-    line_bytes = sum(16 * (1 << l) + 16 * (1 << (l - 1)) + 4 * (1 << (l - 1)) for l in range(L - 1, 4, -1))
-    total = circle_bytes + line_bytes
-    extra = {"kernels": ks, "algorithmic_bytes_per_step": total}
+    line = {l: 16 * (1 << l) + 16 * (1 << (l - 1)) + 4 * (1 << (l - 1)) for l in range(L - 1, 4, -1)}
+    total = circle_bytes + sum(line.values())
+    # the wide launch (2^13-input blocks): the circle fold and the 12 line folds below it (inputs of
+    # log 21 .. 10); its later folds read the block's previous outputs from LDS, so count only their
+    # writes and twiddles
+    wide = circle_bytes + sum(16 * (1 << (l - 1)) + 4 * (1 << (l - 1)) for l in range(L - 1, L - 13, -1))
+    extra = {"kernels": ks, "algorithmic_bytes_per_step": total, "ms_per_step_per_layer_launches": round(ms_layers, 4),
+             "launches_per_step": {"fused_chain": 2, "per_layer": len(layers)}}
     if not args.no_cpu_baseline:
         # one fresh pass on the device (the timed steps accumulate into layer 0) against the oracle
         layers[0] = S.SecureColumn.zeros(1 << (L - 1))
@@ -385,9 +399,9 @@ def bench_fri(args):
                                  "sample": f"whole fold chain; {dt:.3f} s", "cores": _cores()}
         extra["parity_matches_oracle"] = bool(all((layers[i].to_numpy() == outs[i]).all() for i in range(len(layers))))
     return _line("FRI fold chain GB/s (log 22 circle -> line -> log 4)", total / (ms * 1e-3) / 1e9, "GB/s", ms, args,
-                 {"workload": "FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas"},
-                 _roof(circle_bytes, ks["fold_circle_into_line"]["avg_ms"], "fold_circle_into_line",
-                       "fri_fold_2e22_to_2e4"), extra)
+                 {"workload": "FRI: fold_circle_into_line log 22 -> 21, then fold_line to log 4, seeded alphas "
+                              "(fused chain: two launches)"},
+                 _roof(wide, ks["fri_fold_chain"]["avg_ms"], "fri_fold_chain", "fri_fold_2e22_to_2e4"), extra)
 
 
 def bench_quotients(args):

@@ -273,6 +273,17 @@ int stwo_hip_fold_line_async(const uint32_t *itwiddles, size_t twiddle_offset, s
 int stwo_hip_fold_circle_into_line_async(const uint32_t *itwiddles, size_t twiddle_offset, size_t n,
                                          uint32_t *const *eval_values, StwoQM31 alpha,
                                          uint32_t *const *folded_values);
+/* A whole FRI fold sequence whose coefficients are all known (FriOps::fold_circle_into_line then
+ * FriOps::fold_line layer after layer, stwo_gpu_backend/src/fri.rs:15-66): layers[0] accumulates
+ * the circle fold of `circle` (2^log_circle values) with alphas[0], exactly as
+ * fold_circle_into_line(itwiddles, tree_len - 2^(log_circle-1), ...) does, then
+ * layers[j] = fold_line(layers[j-1], alphas[j]) at twiddle offset tree_len - 2^(log_circle-j),
+ * j = 1 .. n_layers-1.  layers: HOST array of 4 * n_layers device pointers (four coordinate
+ * columns per layer, layer j of 2^(log_circle-1-j) values); alphas: HOST.  Each launch folds
+ * 2^13-value ranges up to 13 times in LDS (a log-22 circle to 2^4: two launches). */
+int stwo_hip_fri_fold_chain_async(const uint32_t *itwiddles, uint64_t tree_len, uint32_t log_circle,
+                                  uint32_t *const *circle, uint32_t n_layers, const StwoQM31 *alphas,
+                                  uint32_t *const *layers);
 /* Asynchronous DEEP quotients.  columns: HOST array of n_columns device pointers (each of
  * domain_size values); points/batch_sizes/column_indexes/values: HOST, in batch order;
  * out: HOST array of 4 device columns. */

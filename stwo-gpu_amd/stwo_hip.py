@@ -137,6 +137,7 @@ def _declare(L):
         "stwo_hip_server_limit_work": (C.c_int, [vp, u64]),
         "stwo_hip_gather_rows": (C.c_int, [vp, u32, u64, u32p, u32, u32p]),
         "stwo_hip_eval_at_points_columns": (C.c_int, [vp, u32, u32, vp, u32, vp]),
+        "stwo_hip_fri_fold_chain_async": (C.c_int, [vp, u64, u32, vp, u32, vp, vp]),
         "stwo_hip_eval_at_points_columns_noalloc": (C.c_int, [vp, u32, u32, vp, u32, vp]),
         "stwo_hip_merkle_auth_paths": (C.c_int, [vp, u32, u32p, u32, u32p]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
@@ -620,6 +621,16 @@ def fold_line_async(src: SecureColumn, dst: SecureColumn, alpha, itw_ptr, twiddl
 def fold_circle_into_line_async(dst: SecureColumn, src: SecureColumn, alpha, itw_ptr, twiddle_offset):
     _c(lib().stwo_hip_fold_circle_into_line_async(itw_ptr, twiddle_offset, len(src), src.ptrs(), qm31(alpha),
                                                   dst.ptrs()))
+
+
+def fri_fold_chain_async(circle: SecureColumn, layers, alphas, itw_ptr, tree_len):
+    """layers[0] accumulates the circle fold of `circle` with alphas[0]; layers[j] = fold_line(layers[j-1],
+    alphas[j]) (all coefficients known up front; two launches for a log-22 circle folded to 2^4)."""
+    log_circle = len(circle).bit_length() - 1
+    ptrs = (vp * (4 * len(layers)))(*[p for l in layers for p in l.ptrs()])
+    al = (QM31 * len(alphas))(*[qm31(a) for a in alphas])
+    _c(lib().stwo_hip_fri_fold_chain_async(itw_ptr, tree_len, log_circle, circle.ptrs(), len(layers), C.cast(al, vp),
+                                           C.cast(ptrs, vp)))
 
 
 class QuotientSamples:

@@ -174,6 +174,28 @@ def test_fri_chain_log22_to_log4():
     assert len(layers[-1]) == 16
 
 
+def test_fri_fold_chain_log22_to_log4():
+    # the same chain through the fused entry point (two launches)
+    L = 22
+    rng = np.random.default_rng(0x5EED0002)
+    src = S.SecureColumn([_device_uniform(1 << L, 0x5EED0002, (2 << 40) + k * (1 << L)) for k in range(4)])
+    layers = [S.SecureColumn.zeros(1 << l) for l in range(L - 1, 3, -1)]
+    root = S.canonic_half_coset(L)
+    tw, itw = S.cached_twiddles(root)
+    alphas = [S.random_qm31(rng) for _ in range(len(layers))]
+    S.fri_fold_chain_async(src, layers, alphas, itw, 1 << root.log_size)
+    S.synchronize()
+    host = np.stack([O.fill_uniform_m31(1 << L, 0x5EED0002, (2 << 40) + k * (1 << L)) for k in range(4)])
+    cur = O.fold_circle_into_line(np.zeros((4, 1 << (L - 1)), np.uint32), host, root.initial_index, root.step_size,
+                                  alphas[0])
+    assert (layers[0].to_numpy() == cur).all(), "circle fold"
+    d = S.Coset(root.initial_index, root.step_size, root.log_size)
+    for i in range(1, len(layers)):
+        cur = O.fold_line(cur, d.initial_index, d.step_size, alphas[i])
+        d = d.double()
+        assert (layers[i].to_numpy() == cur).all(), f"line layer {i} (log {L - 1 - i})"
+
+
 # ------------------------------------------------------------------ quotients 200 x 2^21, 3 points
 def test_quotients_200_columns_log21_3_points():
     L, C = 21, 200

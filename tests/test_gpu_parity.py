@@ -326,6 +326,32 @@ def test_fold_circle_into_line(log_n):
     assert (dst.to_numpy() == O.fold_circle_into_line(dst0, src, hc.initial_index, hc.step_size, alpha)).all()
 
 
+@pytest.mark.parametrize("log_circle,n_layers,bigger_tree", [
+    (2, 1, False), (2, 2, False), (3, 3, False), (5, 4, True), (13, 13, False), (14, 14, True),
+    (16, 16, False), (17, 12, False), (20, 17, True)])
+def test_fri_fold_chain(log_circle, n_layers, bigger_tree):
+    # fold_circle_into_line then fold_line per layer, every coefficient known up front; the
+    # cases cross the chain's launch boundaries (2^13-value block ranges, 512-block minimum)
+    rng = np.random.default_rng(3000 + log_circle)
+    src = _rand_secure(1 << log_circle, 3100 + log_circle)
+    dst0 = _rand_secure(1 << (log_circle - 1), 3200 + log_circle)
+    alphas = [tuple(int(x) for x in rng.integers(0, S.P, 4)) for _ in range(n_layers)]
+    hc = S.canonic_half_coset(log_circle)
+    root = S.canonic_half_coset(log_circle + 1) if bigger_tree else hc  # a larger tree serves it too
+    tw, itw = S.cached_twiddles(root)
+    layers = [S.SecureColumn.from_numpy(dst0)] + [S.SecureColumn.zeros(1 << (log_circle - 1 - j))
+                                                   for j in range(1, n_layers)]
+    S.fri_fold_chain_async(S.SecureColumn.from_numpy(src), layers, alphas, itw, 1 << root.log_size)
+    S.synchronize()
+    cur = O.fold_circle_into_line(dst0, src, hc.initial_index, hc.step_size, alphas[0])
+    assert (layers[0].to_numpy() == cur).all(), "circle fold"
+    d = S.Coset(hc.initial_index, hc.step_size, hc.log_size)
+    for j in range(1, n_layers):
+        cur = O.fold_line(cur, d.initial_index, d.step_size, alphas[j])
+        d = d.double()
+        assert (layers[j].to_numpy() == cur).all(), f"line layer {j}"
+
+
 def test_fold_circle_into_line_fibonacci_fixture():
     f = FIX["fold_circle_into_line_fibonacci"]
     rc = f["root_coset"]
