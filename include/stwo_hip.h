@@ -372,6 +372,13 @@ int stwo_hip_wide_fib_proof_size(const StwoWideFibParams *params, uint64_t *proo
 /* One proof on the library stream; returns when the proof is in `proof`. */
 int stwo_hip_wide_fib_prove(const StwoWideFibParams *params, uint64_t seed, uint32_t *proof,
                             uint64_t proof_capacity_words);
+/* n_jobs (1 .. 16) proofs of one statement shape from seeds[0 .. n_jobs-1], proved together:
+ * every phase (trace, LDE, Merkle commitments with their channel steps, composition, OODS
+ * evaluation, quotients, FRI folds, proof of work, decommitment gather) is one launch for all
+ * of them.  proofs: n_jobs x proof_words words, job k's at k * proof_words, each the proof
+ * stwo_hip_wide_fib_prove gives for that seed.  No counterpart in the reference. */
+int stwo_hip_wide_fib_prove_batch(const StwoWideFibParams *params, const uint64_t *seeds, uint32_t n_jobs,
+                                  uint32_t *proofs, uint64_t proof_capacity_words);
 
 /* Proof server with continuous batching: jobs are admitted in arrival order while fewer than
  * max_inflight run and their device footprints fit hbm_budget_bytes; each runs on its own
@@ -385,6 +392,11 @@ int stwo_hip_server_create(StwoProofServer **server, uint64_t hbm_budget_bytes, 
  * flight plus its own exceed max_trace_elements (0: no limit, the default); set before
  * submitting.  No counterpart in the reference (it has no server). */
 int stwo_hip_server_limit_work(StwoProofServer *server, uint64_t max_trace_elements);
+/* Same-phase batching across jobs: admitting a job of at most max_batch_work trace elements
+ * also takes up to max_batch - 1 queued jobs of the same statement shape, and they are proved
+ * as one batch (as stwo_hip_wide_fib_prove_batch; each result is the job's own proof).
+ * max_batch 1 (the default) turns it off; at most 16.  Set before submitting. */
+int stwo_hip_server_batch(StwoProofServer *server, uint32_t max_batch, uint64_t max_batch_work);
 int stwo_hip_server_submit(StwoProofServer *server, const StwoWideFibParams *params, uint64_t seed,
                            uint64_t *job_id);
 int stwo_hip_server_wait(StwoProofServer *server, uint32_t timeout_us, uint64_t *job_id, int *job_status,

@@ -74,19 +74,20 @@ __device__ __forceinline__ void compress(uint32_t (&h)[8], const uint32_t (&m)[1
 }
 
 // Absorb the columns of one layer at `row` (16-column chunks, zero-padded remainder).
+// jo: the job offset of the column pointers (see jptr)
 __device__ __forceinline__ void absorb_columns_simple(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
-                                                      uint32_t row) {
+                                                      uint32_t row, uint64_t jo) {
   uint32_t c = 0;
   for (; c + 16 <= n_cols; c += 16) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = cols[c + j][row];
+    for (int j = 0; j < 16; j++) m[j] = jptr(cols[c + j], jo)[row];
     compress(h, m);
   }
   if (c < n_cols) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? cols[c + j][row] : 0u;
+    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? jptr(cols[c + j], jo)[row] : 0u;
     compress(h, m);
   }
 }
@@ -106,18 +107,20 @@ __device__ __forceinline__ void load_children(const Hash* prev, uint32_t i, uint
 }
 
 // One layer, one lane per node (commit_on_first_layer / commit_on_layer_with_previous).
+// js: byte stride between the jobs of a batched launch (blockIdx.z = job; 0 for one job)
 __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_cols, const uint32_t* const* cols,
-                                                    const Hash* prev, Hash* out) {
+                                                    const Hash* prev, Hash* out, uint64_t js) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= size) return;
+  const uint64_t jo = job_offset(js);
   uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (prev) {
     uint32_t m[16];
-    load_children(prev, i, m);
+    load_children(jptr(prev, jo), i, m);
     compress(h, m);
   }
-  absorb_columns_simple(h, cols, n_cols, i);
-  store_hash(out, i, h);
+  absorb_columns_simple(h, cols, n_cols, i, jo);
+  store_hash(jptr(out, jo), i, h);
 }
 
 
@@ -131,17 +134,20 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
 constexpr uint32_t QUAD_STAGED_MAX_COLS = 4096;  // 32 KiB of pointers
 template <bool STAGED>
 __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t n_cols, const uint32_t* const* cols,
-                                                         const Hash* prev, Hash* out) {
+                                                         const Hash* prev, Hash* out, uint64_t js) {
   __shared__ uint32_t msg[64][16];
   extern __shared__ const uint32_t* sp[];  // STAGED: the n_cols column pointers
   if constexpr (STAGED) {
     for (uint32_t j = threadIdx.x; j < n_cols; j += blockDim.x) sp[j] = cols[j];
     __syncthreads();
   }
+  const uint64_t jo = job_offset(js);
   auto col_ptr = [&](uint32_t col) -> const uint32_t* {
-    if constexpr (STAGED) return sp[col];
-    else return cols[col];
+    if constexpr (STAGED) return jptr(sp[col], jo);
+    else return jptr(cols[col], jo);
   };
+  if (prev) prev = jptr(prev, jo);
+  out = jptr(out, jo);
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t i = t >> 2, q = t & 3, lq = threadIdx.x >> 2;
   if (i >= size) return;  // whole quads leave together
@@ -202,12 +208,13 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
 template <int NT>
 __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uint32_t depth, uint32_t nodes0,
                                              uint32_t base0, Hash* const* outs, const LayerCols* inj, Hash* s,
-                                             uint32_t (*qmsg)[16]) {
+                                             uint32_t (*qmsg)[16], uint64_t jo) {
   for (uint32_t d = 0; d < depth; d++) {
     const uint32_t layer = top - d;
     const uint32_t cnt = nodes0 >> d;
     const uint32_t base = base0 >> d;
     const LayerCols lc = inj[layer];
+    Hash* const out = jptr(outs[layer], jo);
     if (d > 0 && cnt <= NT / 4) {
       // narrow level: one quad per node, children read from s in place
       const uint32_t i = threadIdx.x >> 2, q = threadIdx.x & 3;
@@ -220,13 +227,13 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const uint32_t col = c + 4 * q + k;
-            mb[4 * q + k] = col < lc.n_cols ? lc.cols[col][base + i] : 0u;
+            mb[4 * q + k] = col < lc.n_cols ? jptr(lc.cols[col], jo)[base + i] : 0u;
           }
           __builtin_amdgcn_wave_barrier();
           quad_compress(h_lo, h_hi, mb, q);
         }
-        outs[layer][base + i].s[q] = h_lo;
-        outs[layer][base + i].s[4 + q] = h_hi;
+        out[base + i].s[q] = h_lo;
+        out[base + i].s[4 + q] = h_hi;
       }
       __syncthreads();  // all reads of s for this layer are done
       if (i < cnt) {
@@ -255,8 +262,8 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
           }
           compress(h, m);
         }
-        absorb_columns_simple(h, lc.cols, lc.n_cols, base + i);
-        store_hash(outs[layer], base + i, h);
+        absorb_columns_simple(h, lc.cols, lc.n_cols, base + i, jo);
+        store_hash(out, base + i, h);
       }
       __syncthreads();  // this round's reads of s are done
       if (i < cnt) {
@@ -290,12 +297,21 @@ constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead o
 template <int NT>
 __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
                                                      Hash* const* outs, const LayerCols* inj, RootDraw rd,
-                                                     uint32_t* arrive) {
+                                                     uint32_t* arrive, uint64_t js) {
   extern __shared__ Hash s[];  // max(2^w_log, block count / 2) hashes
   __shared__ uint32_t qmsg[NT / 4][16];  // quad mode: staging of injected column chunks
   __shared__ uint32_t is_last;
+  // batched jobs (blockIdx.z): every arena pointer moves by the job's offset; one arrival
+  // counter per job
+  const uint64_t jo = job_offset(js);
+  if (prev) prev = jptr(prev, jo);
+  if (arrive) arrive += blockIdx.z;
+  if (rd.ch) {
+    rd.ch = jptr(rd.ch, jo);
+    rd.dst = jptr(rd.dst, jo);
+  }
   const uint32_t bd = arrive ? w_log + 1 : depth;  // levels built by every block
-  build_levels<NT>(prev, top, bd, 1u << w_log, blockIdx.x << w_log, outs, inj, s, qmsg);
+  build_levels<NT>(prev, top, bd, 1u << w_log, blockIdx.x << w_log, outs, inj, s, qmsg, jo);
   if (arrive) {
     __threadfence();  // this block's nodes are visible device-wide before it counts in
     __syncthreads();
@@ -305,7 +321,7 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
     __threadfence();       // every other block's nodes are visible to the loads below
     if (threadIdx.x == 0) *arrive = 0u;
     const uint32_t t_top = top - bd;  // the block roots are layer t_top + 1
-    build_levels<NT>(outs[t_top + 1], t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg);
+    build_levels<NT>(jptr(outs[t_top + 1], jo), t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg, jo);
   }
   // the launch that wrote the root (layer 0, held in s[0] after the last barrier) runs
   // the channel step on one lane quad: Channel::mix8 and Channel::draw_qm31 / draw_point
@@ -321,11 +337,11 @@ constexpr uint32_t HEAVY_CHUNKS = 2;
 constexpr uint32_t QUAD_LAYER_MAX = 14;  // layers of at most 2^14 nodes are hashed by quads
 constexpr int32_t WIDE_LAYER = 16;
 
-size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log) {
+size_t MerklePlan::table_bytes(uint32_t n_cols, uint32_t max_log, uint32_t n_jobs) {
   const size_t cols = ((size_t)std::max<uint32_t>(n_cols, 1) * sizeof(void*) + 15) & ~(size_t)15;
   const size_t lcs = ((size_t)(max_log + 1) * sizeof(LayerCols) + 15) & ~(size_t)15;
   const size_t outs = ((size_t)(max_log + 1) * sizeof(Hash*) + 15) & ~(size_t)15;
-  return cols + lcs + outs + (size_t)(max_log + 1) * sizeof(uint32_t);
+  return cols + lcs + outs + (size_t)(max_log + 1) * n_jobs * sizeof(uint32_t);
 }
 
 MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
@@ -340,9 +356,10 @@ MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint3
 }
 
 MerklePlan::MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
-                       uint32_t max_log_, void* dev, void* host_image)
-    : max_log(max_log_) {
+                       uint32_t max_log_, void* dev, void* host_image, uint32_t n_jobs_, uint64_t jstride_)
+    : max_log(max_log_), n_jobs(n_jobs_), jstride(jstride_) {
   require(max_log <= 31, "merkle: log size too large");
+  require(n_jobs >= 1 && n_jobs <= 65535, "merkle: bad job count");
   build(n_cols, cols, logs, layers, (char*)dev, (char*)host_image);
 }
 
@@ -374,7 +391,7 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   // arrival counters of the subtree launches that finish in their last block, by top layer;
   // zero at upload, and each launch leaves its counter at zero
   const size_t o_arrive = o_outs + (((size_t)(max_log + 1) * sizeof(Hash*) + 15) & ~(size_t)15);
-  memset(host + o_arrive, 0, (size_t)(max_log + 1) * sizeof(uint32_t));
+  memset(host + o_arrive, 0, (size_t)(max_log + 1) * n_jobs * sizeof(uint32_t));
   dlcs = (const LayerCols*)(dev + o_lcs);
   douts = (Hash* const*)(dev + o_outs);
   darrive = (uint32_t*)(dev + o_arrive);
@@ -425,13 +442,16 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
         {
           const uint32_t nc = lcs[k].n_cols, blocks = blocks_for(4ull << k, 256);
           if (nc >= 32 && nc <= QUAD_STAGED_MAX_COLS)
-            quad_layer_kernel<true><<<blocks, 256, nc * sizeof(void*), st>>>(1u << k, nc, lcs[k].cols, prev, outs[k]);
+            quad_layer_kernel<true><<<dim3(blocks, 1, n_jobs), 256, nc * sizeof(void*), st>>>(
+                1u << k, nc, lcs[k].cols, prev, outs[k], jstride);
           else
-            quad_layer_kernel<false><<<blocks, 256, 0, st>>>(1u << k, nc, lcs[k].cols, prev, outs[k]);
+            quad_layer_kernel<false><<<dim3(blocks, 1, n_jobs), 256, 0, st>>>(1u << k, nc, lcs[k].cols, prev,
+                                                                              outs[k], jstride);
         }
         else
-          layer_kernel<<<blocks_for(1ull << k, 256), 256, 0, st>>>(1u << k, lcs[k].n_cols, lcs[k].cols, prev,
-                                                                     outs[k]);
+          layer_kernel<<<dim3(blocks_for(1ull << k, 256), 1, n_jobs), 256, 0, st>>>(1u << k, lcs[k].n_cols,
+                                                                                     lcs[k].cols, prev, outs[k],
+                                                                                     jstride);
       });
     } else {
       // narrow runs: 2^SUB_BLOCK_LOG nodes of layer top per block (the whole layer when it
@@ -450,9 +470,9 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
         drawn = true;
       }
       timed("merkle_subtree", [&] {
-        subtree_kernel<SUB_NT><<<blocks, SUB_NT, lds, st>>>(s.top == max_log ? nullptr : outs[s.top + 1], s.top,
-                                                            s.depth, w_log, douts, dlcs, rd,
-                                                            tail ? darrive + s.top : nullptr);
+        subtree_kernel<SUB_NT><<<dim3(blocks, 1, n_jobs), SUB_NT, lds, st>>>(
+            s.top == max_log ? nullptr : outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs, rd,
+            tail ? darrive + (size_t)s.top * n_jobs : nullptr, jstride);
       });
     }
   }
@@ -480,7 +500,7 @@ extern "C" void commit_on_first_layer(size_t size, size_t amount_of_columns, uin
     if (!size) return;
     timed("merkle_leaf", [&] {
       merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(
-          (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, nullptr, (Hash*)result);
+          (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, nullptr, (Hash*)result, 0);
     });
     sync();
   });
@@ -495,7 +515,7 @@ extern "C" void commit_on_layer_with_previous(size_t size, size_t amount_of_colu
     timed("merkle_node", [&] {
       merkle::layer_kernel<<<merkle::blocks_for(size, 256), 256, 0, ctx().cur>>>(
           (uint32_t)size, (uint32_t)amount_of_columns, (const uint32_t* const*)columns, (const Hash*)previous_layer,
-          (Hash*)result);
+          (Hash*)result, 0);
     });
     sync();
   });

@@ -72,7 +72,11 @@ __global__ __launch_bounds__(NT) void reduce_pass(const qm31* in, uint32_t g, Fa
 }
 
 // ---- many columns of one size at one point held in device memory ----
-__global__ void factors_kernel(const qm31* __restrict__ point, uint32_t L, Factors* F) {
+// js (here and below): byte stride between the jobs of a batched launch (blockIdx.z = job)
+__global__ void factors_kernel(const qm31* __restrict__ point, uint32_t L, Factors* F, uint64_t js) {
+  const uint64_t jo = job_offset(js);
+  point = jptr(point, jo);
+  F = jptr(F, jo);
   qm31 x = point[0];
   F->f[L - 1] = point[1];
   for (uint32_t i = 2; i <= L; i++) {
@@ -83,9 +87,13 @@ __global__ void factors_kernel(const qm31* __restrict__ point, uint32_t L, Facto
 
 // block (b, col) folds coefficients [b 2^cb, (b+1) 2^cb) of column col
 __global__ __launch_bounds__(NT) void first_pass_cols(const uint32_t* const* __restrict__ cols, uint32_t L,
-                                                      uint32_t cb, const Factors* __restrict__ F, qm31* partial) {
+                                                      uint32_t cb, const Factors* __restrict__ F, qm31* partial,
+                                                      uint64_t js) {
   __shared__ qm31 s[1 << (CB - 1)];
-  const uint32_t* c = cols[blockIdx.y];
+  const uint64_t jo = job_offset(js);
+  F = jptr(F, jo);
+  partial = jptr(partial, jo);
+  const uint32_t* c = jptr(cols[blockIdx.y], jo);
   const uint32_t pairs = 1u << (cb - 1);
   const size_t base = (size_t)blockIdx.x << cb;
   const qm31 fy = F->f[L - 1];
@@ -111,8 +119,12 @@ __global__ __launch_bounds__(NT) void first_pass_cols(const uint32_t* const* __r
 constexpr uint32_t LO_BITS = 12, GEMV_COLS = 8;
 
 __global__ __launch_bounds__(256) void basis_tables_kernel(const Factors* __restrict__ F, uint32_t L, qm31* lo,
-                                                           qm31* hi) {
+                                                           qm31* hi, uint64_t js) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t jo = job_offset(js);
+  F = jptr(F, jo);
+  lo = jptr(lo, jo);
+  hi = jptr(hi, jo);
   const uint32_t n_lo = 1u << LO_BITS, n_hi = 1u << (L - LO_BITS);
   if (t < n_lo) {
     qm31 v = qone();
@@ -142,8 +154,12 @@ __device__ __forceinline__ qm31 wave_sum(qm31 v) {
 
 __global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                    const qm31* __restrict__ lo, const qm31* __restrict__ hi,
-                                                   qm31* __restrict__ partial) {
+                                                   qm31* __restrict__ partial, uint64_t js) {
   __shared__ qm31 wsum[GEMV_COLS][4];
+  const uint64_t jo = job_offset(js);
+  lo = jptr(lo, jo);
+  hi = jptr(hi, jo);
+  partial = jptr(partial, jo);
   const uint32_t h = blockIdx.x, c0 = blockIdx.y * GEMV_COLS, t = threadIdx.x;
   const uint32_t n_hi = gridDim.x;
   qm31 l[16];
@@ -155,7 +171,8 @@ __global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __rest
   const uint32_t off4 = (uint32_t)((((size_t)h << LO_BITS) + 16 * t) >> 2);
   const uint32_t n_here = min(GEMV_COLS, n_cols - c0);
   auto load16 = [&](uint32_t g, uint4 (&q)[4]) {
-    const SH_GLOBAL uint4* p = reinterpret_cast<const SH_GLOBAL uint4*>(as_global(cols[c0 + min(g, n_here - 1)]));
+    const SH_GLOBAL uint4* p =
+        reinterpret_cast<const SH_GLOBAL uint4*>(as_global(jptr(cols[c0 + min(g, n_here - 1)], jo)));
 #pragma unroll
     for (int k = 0; k < 4; k++) q[k] = at(p, off4 + k);
   };
@@ -194,8 +211,11 @@ __global__ __launch_bounds__(256) void gemv_kernel(const uint32_t* const* __rest
 
 // out[c] = sum of the n_hi partials of column c (one block per column)
 __global__ __launch_bounds__(256) void sum_partials_kernel(const qm31* __restrict__ partial, uint32_t n_hi,
-                                                           qm31* __restrict__ out) {
+                                                           qm31* __restrict__ out, uint64_t js) {
   __shared__ qm31 ws[4];
+  const uint64_t jo = job_offset(js);
+  partial = jptr(partial, jo);
+  out = jptr(out, jo);
   const uint32_t c = blockIdx.x, t = threadIdx.x;
   qm31 r = qzero();
   for (uint32_t h = t; h < n_hi; h += 256) r = qadd(r, partial[(size_t)c * n_hi + h]);
@@ -206,8 +226,12 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(const qm31* __restric
 }
 
 __global__ __launch_bounds__(NT) void reduce_pass_dev(const qm31* in, uint32_t g, const Factors* __restrict__ F,
-                                                      int fi, qm31* out) {
+                                                      int fi, qm31* out, uint64_t js) {
   __shared__ qm31 s[1 << (CB - 1)];
+  const uint64_t jo = job_offset(js);
+  in = jptr(in, jo);
+  F = jptr(F, jo);
+  out = jptr(out, jo);
   const uint32_t cnt = 1u << g;
   for (uint32_t i = threadIdx.x; i < cnt; i += NT) s[i] = in[((size_t)blockIdx.x << g) + i];
   __syncthreads();
@@ -225,8 +249,10 @@ size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L) {
 }
 
 // d_out[c] = value at d_point (x, y) of the polynomial with coefficients d_cols[c] (2^L each).
+// n_jobs / js: same-shape jobs of a batched launch, js bytes apart (point, output, scratch and
+// columns all move by the job's offset).
 void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
-                                 qm31* d_out, void* d_scratch) {
+                                 qm31* d_out, void* d_scratch, uint32_t n_jobs, uint64_t js) {
   // coefficient offsets are formed in 32 bits (gmem.hpp): at most 2^30 coefficients
   require(L >= 1 && L <= 30 && n_cols >= 1 && n_cols <= 65535, "eval_at_point_columns: bad sizes");
   eap::Factors* F = (eap::Factors*)d_scratch;
@@ -236,11 +262,12 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
     qm31* hi = lo + (1u << eap::LO_BITS);
     qm31* partial = hi + n_hi;
     timed("eval_at_point", [&] {
-      eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
-      eap::basis_tables_kernel<<<ceil_div((1u << eap::LO_BITS) + n_hi, 256), 256, 0, ctx().cur>>>(F, L, lo, hi);
-      eap::gemv_kernel<<<dim3(n_hi, ceil_div(n_cols, eap::GEMV_COLS)), 256, 0, ctx().cur>>>(d_cols, n_cols, lo, hi,
-                                                                                              partial);
-      eap::sum_partials_kernel<<<n_cols, 256, 0, ctx().cur>>>(partial, n_hi, d_out);
+      eap::factors_kernel<<<dim3(1, 1, n_jobs), 1, 0, ctx().cur>>>(d_point, L, F, js);
+      eap::basis_tables_kernel<<<dim3(ceil_div((1u << eap::LO_BITS) + n_hi, 256), 1, n_jobs), 256, 0, ctx().cur>>>(
+          F, L, lo, hi, js);
+      eap::gemv_kernel<<<dim3(n_hi, ceil_div(n_cols, eap::GEMV_COLS), n_jobs), 256, 0, ctx().cur>>>(
+          d_cols, n_cols, lo, hi, partial, js);
+      eap::sum_partials_kernel<<<dim3(n_cols, 1, n_jobs), 256, 0, ctx().cur>>>(partial, n_hi, d_out, js);
     });
     return;
   }
@@ -250,9 +277,9 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
   qm31* pa = (qm31*)((char*)d_scratch + 512);
   qm31* pb = pa + (size_t)n_cols * blocks;
   timed("eval_at_point", [&] {
-    eap::factors_kernel<<<1, 1, 0, ctx().cur>>>(d_point, L, F);
-    eap::first_pass_cols<<<dim3(blocks, n_cols), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
-                                                                           blocks == 1 ? d_out : pa);
+    eap::factors_kernel<<<dim3(1, 1, n_jobs), 1, 0, ctx().cur>>>(d_point, L, F, js);
+    eap::first_pass_cols<<<dim3(blocks, n_cols, n_jobs), eap::NT, 0, ctx().cur>>>(d_cols, L, cb, F,
+                                                                                   blocks == 1 ? d_out : pa, js);
   });
   int fi = (int)(L - cb) - 1;
   while (blocks > 1) {
@@ -260,7 +287,7 @@ void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols,
     const uint32_t nb = blocks >> g;
     qm31* dst = nb == 1 ? d_out : pb;
     timed("eval_at_point", [&] {
-      eap::reduce_pass_dev<<<nb * n_cols, eap::NT, 0, ctx().cur>>>(pa, g, F, fi, dst);
+      eap::reduce_pass_dev<<<dim3(nb * n_cols, 1, n_jobs), eap::NT, 0, ctx().cur>>>(pa, g, F, fi, dst, js);
     });
     std::swap(pa, pb);
     fi -= (int)g;
@@ -342,7 +369,7 @@ extern "C" int stwo_hip_eval_at_points_columns(const uint32_t* const* columns, u
     // one batched evaluation per point (the scratch is reused in stream order)
     for (uint32_t k = 0; k < n_points; k++)
       eval_at_point_columns_async(table.as<const uint32_t*>(), n_cols, log_size, dpt.as<qm31>() + 2 * k,
-                                  out.as<qm31>() + (size_t)k * n_cols, scratch.p);
+                                  out.as<qm31>() + (size_t)k * n_cols, scratch.p, 1, 0);
     hip_check(hipMemcpyAsync(values, out.p, sizeof(qm31) * n_cols * (size_t)n_points, hipMemcpyDeviceToHost,
                              ctx().cur),
               "hipMemcpyAsync D2H");
@@ -407,7 +434,7 @@ extern "C" int stwo_hip_eval_at_points_columns_noalloc(const uint32_t* const* co
     hip_check(hipMemcpyAsync(dpt, points, 2 * sizeof(qm31) * (size_t)n_points, hipMemcpyHostToDevice, s),
               "hipMemcpyAsync H2D");
     for (uint32_t k = 0; k < n_points; k++)
-      eval_at_point_columns_async(table, n_cols, log_size, dpt + 2 * k, out + (size_t)k * n_cols, work);
+      eval_at_point_columns_async(table, n_cols, log_size, dpt + 2 * k, out + (size_t)k * n_cols, work, 1, 0);
     hip_check(hipMemcpyAsync(values, out, sizeof(qm31) * n_cols * (size_t)n_points, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync D2H");
     sync();

@@ -31,12 +31,22 @@ __device__ __forceinline__ void st(const Soa4& s, size_t i, qm31 v) {
   s.c[3][i] = v.y.b;
 }
 
-// alpha_dev (optional): read alpha from device memory (a challenge drawn on the device)
+// moves the four coordinate columns by a job offset (see jptr)
+__device__ __forceinline__ Soa4 jsoa(Soa4 s, uint64_t jo) {
+  return Soa4{{jptr(s.c[0], jo), jptr(s.c[1], jo), jptr(s.c[2], jo), jptr(s.c[3], jo)}};
+}
+
+// alpha_dev (optional): read alpha from device memory (a challenge drawn on the device).
+// js: byte stride between the jobs of a batched launch (blockIdx.z = job; 0 for one job).
 __global__ __launch_bounds__(256) void fold_line_kernel(const uint32_t* __restrict__ itw, uint32_t half_n,
-                                                        Soa4 src, qm31 alpha, const qm31* alpha_dev, Soa4 dst) {
+                                                        Soa4 src, qm31 alpha, const qm31* alpha_dev, Soa4 dst,
+                                                        uint64_t js) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= half_n) return;
-  if (alpha_dev) alpha = *alpha_dev;
+  const uint64_t jo = job_offset(js);
+  src = jsoa(src, jo);
+  dst = jsoa(dst, jo);
+  if (alpha_dev) alpha = *jptr(alpha_dev, jo);
   uint32_t xi = itw[i];
   qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
   qm31 f0 = qadd(a, b), f1 = qscale(qsub(a, b), xi);
@@ -46,10 +56,13 @@ __global__ __launch_bounds__(256) void fold_line_kernel(const uint32_t* __restri
 // accumulate = false: dst is overwritten with the fold (as if it held zeros)
 __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __restrict__ tw, uint32_t half_n,
                                                           Soa4 src, qm31 alpha, const qm31* alpha_dev,
-                                                          bool accumulate, Soa4 dst) {
+                                                          bool accumulate, Soa4 dst, uint64_t js) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= half_n) return;
-  if (alpha_dev) alpha = *alpha_dev;
+  const uint64_t jo = job_offset(js);
+  src = jsoa(src, jo);
+  dst = jsoa(dst, jo);
+  if (alpha_dev) alpha = *jptr(alpha_dev, jo);
   uint32_t yi;
   if (half_n >= 4) {
     // circle twiddle i: [y, -y, -x, x] over pairs (x, y) of the first line level
@@ -213,7 +226,8 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
 }  // namespace fri
 
 void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src,
-                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate);
+                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate,
+                                 uint32_t n_jobs, uint64_t js);
 
 // layers: n_layers x 4 device pointers (host array), layer j of 2^(log_circle - 1 - j) values.
 void fri_fold_chain_async(const uint32_t* itw, uint64_t tree_len, uint32_t log_circle, uint32_t* const* circle,
@@ -226,7 +240,7 @@ void fri_fold_chain_async(const uint32_t* itw, uint64_t tree_len, uint32_t log_c
   uint32_t* const* cur = circle;
   if (log_circle <= 2) {  // circle domains of 2 and 4 points: their twiddles come from the domain itself
     fold_circle_into_line_async(itw, tree_len - (1ull << (log_circle - 1)), 1ull << log_circle, circle, alphas[0],
-                                layers, nullptr, true);
+                                layers, nullptr, true, 1, 0);
     j = 1;
     log_cur--;
     cur = layers;
@@ -258,22 +272,25 @@ void fri_fold_chain_async(const uint32_t* itw, uint64_t tree_len, uint32_t log_c
   }
 }
 
+// n_jobs / js: same-shape jobs of a batched launch, js bytes apart (source, destination and
+// alpha_dev move by the job's offset; the twiddles are shared)
 void fold_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src, qm31 alpha,
-                     uint32_t* const* dst, const qm31* alpha_dev) {
+                     uint32_t* const* dst, const qm31* alpha_dev, uint32_t n_jobs, uint64_t js) {
   require(n >= 2 && is_pow2(n) && n <= (1ull << 31), "fold_line: n must be a power of two >= 2");
   uint32_t half = (uint32_t)(n >> 1);
   timed("fold_line", [&] {
-    fri::fold_line_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(itw + twiddle_offset, half, fri::soa(src),
-                                                                           alpha, alpha_dev, fri::soa(dst));
+    fri::fold_line_kernel<<<dim3(ceil_div(half, 256), 1, n_jobs), 256, 0, ctx().cur>>>(
+        itw + twiddle_offset, half, fri::soa(src), alpha, alpha_dev, fri::soa(dst), js);
   });
 }
 void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src,
-                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate) {
+                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate,
+                                 uint32_t n_jobs, uint64_t js) {
   require(n >= 2 && is_pow2(n) && n <= (1ull << 31), "fold_circle_into_line: n must be a power of two >= 2");
   uint32_t half = (uint32_t)(n >> 1);
   timed("fold_circle_into_line", [&] {
-    fri::fold_circle_kernel<<<ceil_div(half, 256), 256, 0, ctx().cur>>>(
-        itw + twiddle_offset, half, fri::soa(src), alpha, alpha_dev, accumulate, fri::soa(dst));
+    fri::fold_circle_kernel<<<dim3(ceil_div(half, 256), 1, n_jobs), 256, 0, ctx().cur>>>(
+        itw + twiddle_offset, half, fri::soa(src), alpha, alpha_dev, accumulate, fri::soa(dst), js);
   });
 }
 
@@ -285,7 +302,7 @@ extern "C" int stwo_hip_fold_line_async(const uint32_t* itwiddles, size_t twiddl
                                         uint32_t* const* eval_values, StwoQM31 alpha, uint32_t* const* folded_values) {
   return guarded([&] {
     fold_line_async(itwiddles, twiddle_offset, n, eval_values, qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}},
-                    folded_values, nullptr);
+                    folded_values, nullptr, 1, 0);
   });
 }
 
@@ -294,7 +311,7 @@ extern "C" int stwo_hip_fold_circle_into_line_async(const uint32_t* itwiddles, s
                                                     uint32_t* const* folded_values) {
   return guarded([&] {
     fold_circle_into_line_async(itwiddles, twiddle_offset, n, eval_values,
-                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true);
+                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true, 1, 0);
   });
 }
 
@@ -314,7 +331,7 @@ extern "C" void fold_line(const uint32_t* gpu_domain, size_t twiddle_offset, siz
                           StwoQM31 alpha, uint32_t* const* folded_values) {
   guarded([&] {
     fold_line_async(gpu_domain, twiddle_offset, n, eval_values, qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}},
-                    folded_values, nullptr);
+                    folded_values, nullptr, 1, 0);
     sync();
   });
 }
@@ -323,7 +340,7 @@ extern "C" void fold_circle_into_line(const uint32_t* gpu_domain, size_t twiddle
                                       uint32_t* const* eval_values, StwoQM31 alpha, uint32_t* const* folded_values) {
   guarded([&] {
     fold_circle_into_line_async(gpu_domain, twiddle_offset, n, eval_values,
-                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true);
+                                qm31{{alpha.a, alpha.b}, {alpha.c, alpha.d}}, folded_values, nullptr, true, 1, 0);
     sync();
   });
 }

@@ -18,8 +18,12 @@ constexpr uint32_t NT = 256;
 constexpr uint32_t BLOCKS = 2048;  // 8 per CU on 256 CUs
 constexpr uint64_t WIDTH = (uint64_t)NT * BLOCKS;
 
+// js: byte stride between the jobs of a batched launch (blockIdx.z = job; 0 for one job)
 __global__ __launch_bounds__(NT) void grind_kernel(const uint32_t* __restrict__ digest, uint32_t pow_bits,
-                                                   uint32_t max_rounds, unsigned long long* best) {
+                                                   uint32_t max_rounds, unsigned long long* best, uint64_t js) {
+  const uint64_t jo = job_offset(js);
+  digest = jptr(digest, jo);
+  best = jptr(best, jo);
   uint32_t d[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) d[i] = digest[i];
@@ -34,19 +38,20 @@ __global__ __launch_bounds__(NT) void grind_kernel(const uint32_t* __restrict__ 
   }
 }
 
-__global__ void init_kernel(unsigned long long* best) { *best = ~0ull; }
+__global__ void init_kernel(unsigned long long* best, uint64_t js) { *jptr(best, job_offset(js)) = ~0ull; }
 
 }  // namespace grind
 
 // Result written to *d_nonce (device memory, ~0 when no nonce below max_rounds * WIDTH).
-void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce) {
+// n_jobs / js: same-shape jobs of a batched launch, js bytes apart.
+void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce, uint32_t n_jobs, uint64_t js) {
   require(pow_bits <= 40, "grind: at most 40 bits of work");
   const uint64_t need = (1ull << pow_bits) * 64;  // ~e^-64 chance of no hit
   const uint32_t rounds = (uint32_t)std::max<uint64_t>(1, (need + grind::WIDTH - 1) / grind::WIDTH);
   timed("grind", [&] {
-    grind::init_kernel<<<1, 1, 0, ctx().cur>>>((unsigned long long*)d_nonce);
-    grind::grind_kernel<<<grind::BLOCKS, grind::NT, 0, ctx().cur>>>(d_digest, pow_bits, rounds,
-                                                                     (unsigned long long*)d_nonce);
+    grind::init_kernel<<<dim3(1, 1, n_jobs), 1, 0, ctx().cur>>>((unsigned long long*)d_nonce, js);
+    grind::grind_kernel<<<dim3(grind::BLOCKS, 1, n_jobs), grind::NT, 0, ctx().cur>>>(
+        d_digest, pow_bits, rounds, (unsigned long long*)d_nonce, js);
   });
 }
 
@@ -59,7 +64,7 @@ extern "C" int stwo_hip_grind(const uint32_t* digest, uint32_t pow_bits, uint64_
     require(digest != nullptr && nonce != nullptr, "grind: null argument");
     DevBuf d = upload(digest, 32);
     DevBuf out(8);
-    grind_async(d.as<uint32_t>(), pow_bits, out.as<uint64_t>());
+    grind_async(d.as<uint32_t>(), pow_bits, out.as<uint64_t>(), 1, 0);
     uint64_t r = 0;
     hip_check(hipMemcpyAsync(&r, out.p, 8, hipMemcpyDeviceToHost, ctx().cur), "hipMemcpy");
     sync();

@@ -39,16 +39,20 @@ struct MerklePlan {
     uint32_t top, depth;
   };
   uint32_t max_log;
+  // jobs hashed by every launch (blockIdx.z): n_jobs trees of the same shape whose buffers lie
+  // jstride bytes apart (the proof jobs of a batch); the tables hold job 0's pointers
+  uint32_t n_jobs = 1;
+  uint64_t jstride = 0;
   DevBuf own;                     // the device tables, unless they live in caller memory
   const LayerCols* dlcs = nullptr;  // device: per-layer column tables
   Hash* const* douts = nullptr;     // device: layer output pointers
-  uint32_t* darrive = nullptr;      // device: per-layer arrival counters of subtree launches
+  uint32_t* darrive = nullptr;      // device: arrival counters of subtree launches, [layer][job]
   std::vector<LayerCols> lcs;       // host copies (device column-table pointers)
   std::vector<Hash*> outs;
   std::vector<Step> steps;
 
   // Bytes of device tables for a tree with n_cols columns and layers 0 .. max_log.
-  static size_t table_bytes(uint32_t n_cols, uint32_t max_log);
+  static size_t table_bytes(uint32_t n_cols, uint32_t max_log, uint32_t n_jobs = 1);
 
   // cols/logs: host arrays (device column pointers); layers: host array of max_log + 1
   // device buffers, layers[k] holding 2^k hashes.  The tables are uploaded to a buffer
@@ -57,8 +61,9 @@ struct MerklePlan {
              uint32_t max_log);
   // Same, with the tables placed at device address `dev` (table_bytes long); their bytes
   // are written to `host_image` and the caller uploads them to `dev` before enqueue().
+  // n_jobs / jstride: see above (table_bytes(n_cols, max_log, n_jobs) bytes at dev).
   MerklePlan(uint32_t n_cols, const uint32_t* const* cols, const uint32_t* logs, Hash* const* layers,
-             uint32_t max_log, void* dev, void* host_image);
+             uint32_t max_log, void* dev, void* host_image, uint32_t n_jobs = 1, uint64_t jstride = 0);
   // On ctx().cur; allocates nothing.  layer_ready (optional, indexed by layer): an event
   // per layer (or null) that marks its injected columns written, e.g. on another stream;
   // each step waits only for the layers it reads, so hashing starts while other

@@ -39,24 +39,27 @@
 namespace sh {
 
 // other modules
-void wide_fib_trace_async(uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_n, const uint64_t* d_seed);
-void vanishing_inverse_async(uint32_t* d_out, uint32_t log_ext, uint32_t log_n);
-void qm31_powers_async(const qm31* d_alpha, uint32_t n, uint32_t* d_powers);
+// (n_jobs, js): same-shape jobs of a batched launch whose buffers lie js bytes apart
+void wide_fib_trace_async(uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_n, const uint64_t* d_seed,
+                          uint32_t n_jobs, uint64_t js);
+void vanishing_inverse_async(uint32_t* d_out, uint32_t log_ext, uint32_t log_n, uint32_t n_jobs, uint64_t js);
+void qm31_powers_async(const qm31* d_alpha, uint32_t n, uint32_t* d_powers, uint32_t n_jobs, uint64_t js);
 void wide_fib_quotients_async(uint32_t* const q[4], const uint32_t* const* d_trace, const uint32_t* powers,
-                              const uint32_t* dinv, uint32_t n_rows, uint32_t n_cols);
+                              const uint32_t* dinv, uint32_t n_rows, uint32_t n_cols, uint32_t n_jobs, uint64_t js);
 size_t eval_at_point_columns_scratch_bytes(uint32_t n_cols, uint32_t L);
 void eval_at_point_columns_async(const uint32_t* const* d_cols, uint32_t n_cols, uint32_t L, const qm31* d_point,
-                                 qm31* d_out, void* d_scratch);
+                                 qm31* d_out, void* d_scratch, uint32_t n_jobs, uint64_t js);
 size_t quotient_single_point_scratch_bytes(uint32_t n_cols, uint32_t max_log);
 void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
                                              const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
                                              const qm31* d_values, const qm31* d_rc, void* d_scratch,
-                                             uint32_t* const out[4]);
+                                             uint32_t* const out[4], uint32_t n_jobs, uint64_t js);
 void fold_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src, qm31 alpha,
-                     uint32_t* const* dst, const qm31* alpha_dev);
+                     uint32_t* const* dst, const qm31* alpha_dev, uint32_t n_jobs, uint64_t js);
 void fold_circle_into_line_async(const uint32_t* itw, size_t twiddle_offset, size_t n, uint32_t* const* src,
-                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate);
-void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce);
+                                 qm31 alpha, uint32_t* const* dst, const qm31* alpha_dev, bool accumulate,
+                                 uint32_t n_jobs, uint64_t js);
+void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce, uint32_t n_jobs, uint64_t js);
 
 namespace prover {
 
@@ -78,6 +81,11 @@ struct JobBlock {
   uint32_t queries[MAX_QUERIES];
 };
 
+// the trace seeds of a batch, by job
+struct Seeds {
+  uint64_t s[MAX_BATCH];
+};
+
 struct GatherOp {
   const uint32_t* src;
   uint32_t dst;
@@ -87,21 +95,26 @@ struct GatherOp {
   uint16_t pad;
 };
 
-// ---- channel steps (one lane) ----
-__global__ void k_begin(JobBlock* jb, uint64_t seed, Params p, uint32_t n_line) {
-  jb->seed = seed;
+// ---- channel steps (one lane per job; blockIdx.z = job, js = byte stride between jobs) ----
+__global__ void k_begin(JobBlock* jb, Seeds seeds, Params p, uint32_t n_line, uint64_t js) {
+  jb = jptr(jb, job_offset(js));
+  jb->seed = seeds.s[blockIdx.z];
   const uint32_t h[8] = {MAGIC, VERSION, p.n_cols, p.log_n, p.n_queries, p.pow_bits, p.last_log, n_line};
   for (int i = 0; i < 8; i++) jb->header[i] = h[i];
   jb->ch.reset();
   const uint32_t st[6] = {p.n_cols, p.log_n, p.n_queries, p.pow_bits, p.last_log, STATEMENT_TAG};
   jb->ch.mix(st, 6);
 }
-__global__ void k_mix_root_draw(JobBlock* jb, const merkle::Hash* root, qm31* dst) {
-  jb->ch.mix8(root->s);
-  *dst = jb->ch.draw_qm31();
+__global__ void k_mix_root_draw(JobBlock* jb, const merkle::Hash* root, qm31* dst, uint64_t js) {
+  const uint64_t jo = job_offset(js);
+  jb = jptr(jb, jo);
+  jb->ch.mix8(jptr(root, jo)->s);
+  *jptr(dst, jo) = jb->ch.draw_qm31();
 }
-__global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root) {
-  jb->ch.mix8(root->s);
+__global__ void k_mix_root_draw_point(JobBlock* jb, const merkle::Hash* root, uint64_t js) {
+  const uint64_t jo = job_offset(js);
+  jb = jptr(jb, jo);
+  jb->ch.mix8(jptr(root, jo)->s);
   jb->ch.draw_point(jb->z[0], jb->z[1]);
 }
 // Tree root (blake2s.hpp) of a QM31 vector held as AoS (soa == false: v[i] = 4 words) or as
@@ -173,9 +186,13 @@ __device__ const merkle::Hash* block_tree_root(const qm31* aos, Soa4c soa, bool 
 
 // mix the tree root of the OODS samples, draw the quotient coefficient and the first FRI
 // folding coefficient (Channel::mix8, two Channel::draw_qm31), on one quad
-__global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n) {
+__global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, const qm31* samples, uint32_t n,
+                                                             uint64_t js) {
   __shared__ merkle::Hash buf[TREE_MAX_LEAVES], buf2[TREE_MAX_LEAVES / 2];
   __shared__ uint32_t qm[TREE_NQ][16];
+  const uint64_t jo = job_offset(js);
+  jb = jptr(jb, jo);
+  samples = jptr(samples, jo);
   const merkle::Hash* root = block_tree_root(samples, Soa4c{}, false, n, buf, buf2, qm);
   if (threadIdx.x < 4) {
     const uint32_t q = threadIdx.x;
@@ -193,9 +210,12 @@ __global__ __launch_bounds__(TREE_NT) void k_mix_samples_draw(JobBlock* jb, cons
   }
 }
 // mix the tree root of the last FRI layer (Channel::mix8), on one quad
-__global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n) {
+__global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, uint32_t n, uint64_t js) {
   __shared__ merkle::Hash buf[TREE_MAX_LEAVES], buf2[TREE_MAX_LEAVES / 2];
   __shared__ uint32_t qm[TREE_NQ][16];
+  const uint64_t jo = job_offset(js);
+  jb = jptr(jb, jo);
+  for (int k = 0; k < 4; k++) last.c[k] = jptr(last.c[k], jo);
   const merkle::Hash* root = block_tree_root(nullptr, last, true, n, buf, buf2, qm);
   if (threadIdx.x < 4) {
     const uint32_t q = threadIdx.x;
@@ -206,19 +226,24 @@ __global__ __launch_bounds__(TREE_NT) void k_mix_last(JobBlock* jb, Soa4c last, 
     if (q == 0) jb->ch.n_draws = 0;
   }
 }
-__global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size) {
+__global__ void k_mix_nonce_queries(JobBlock* jb, uint32_t n_queries, uint32_t log_size, uint64_t js) {
+  jb = jptr(jb, job_offset(js));
   const uint32_t w[2] = {(uint32_t)jb->nonce, (uint32_t)(jb->nonce >> 32)};
   jb->ch.mix(w, 2);
   jb->ch.draw_queries(jb->queries, n_queries, log_size);
 }
 
 __global__ __launch_bounds__(256) void k_gather(const GatherOp* __restrict__ ops, uint32_t n_ops,
-                                                const JobBlock* __restrict__ jb, uint32_t* __restrict__ proof) {
+                                                const JobBlock* __restrict__ jb, uint32_t* __restrict__ proof,
+                                                uint64_t js) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_ops) return;
+  const uint64_t jo = job_offset(js);
+  jb = jptr(jb, jo);
+  proof = jptr(proof, jo);
   const GatherOp op = ops[i];
   const uint32_t idx = op.q < 0 ? 0u : ((jb->queries[op.q] >> op.shift) ^ op.xr);
-  const uint32_t* src = op.src + (size_t)idx * op.words;
+  const uint32_t* src = jptr(op.src, jo) + (size_t)idx * op.words;
   for (uint32_t w = 0; w < op.words; w++) proof[op.dst + w] = src[w];
 }
 
@@ -356,13 +381,18 @@ uint32_t half_step(uint32_t log) { return (1u << (32 - log)) & 0x7fffffffu; }
 
 }  // namespace
 
-uint64_t footprint_bytes(const Params& p) {
+// byte stride between the arenas of a batch's jobs
+uint64_t job_stride(const Params& p, const Layout& L) { return (Arena(p, L).end + 4095) & ~4095ull; }
+
+uint64_t footprint_bytes(const Params& p, uint32_t n_jobs) {
   Layout L(p);
-  Arena A(p, L);
-  return A.end + 64 * 1024 * (uint64_t)(3 + L.n_line);  // plus plan tables
+  return job_stride(p, L) * n_jobs + 64 * 1024 * (uint64_t)(3 + L.n_line) +  // plus plan tables
+         (uint64_t)n_jobs * 16 * ((uint64_t)p.n_cols + 4);                      // batch pointer tables
 }
 
-Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream(s) {
+Job::Job(const Params& p_, hipStream_t s, uint32_t n_jobs_)
+    : p((validate(p_), p_)), L(p_), stream(s), n_jobs(n_jobs_) {
+  require(n_jobs >= 1 && n_jobs <= MAX_BATCH, "wide fibonacci prover: batch of 1 .. 16 jobs");
   StreamScope scope(stream);
   const uint32_t C = p.n_cols, ln = p.log_n;
   const uint64_t n = 1ull << ln;
@@ -376,7 +406,8 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   }
 
   Arena A(p, L);
-  arena = DevBuf(A.end);
+  jstride = n_jobs > 1 ? job_stride(p, L) : 0;
+  arena = DevBuf(n_jobs > 1 ? jstride * n_jobs : A.end);
   char* base = arena.as<char>();
   coeffs = (uint32_t*)(base + A.coeffs);
   lde = (uint32_t*)(base + A.lde);
@@ -393,24 +424,36 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   proof = (uint32_t*)(base + A.proof);
   samples = (qm31*)(base + A.samples);
 
-  // device pointer tables: coeffs | lde | comp | comp_lde
+  // device pointer tables: coeffs | lde | comp | comp_lde of job 0, then (a batch of several
+  // jobs) the same four tables over every job, for the LDE launches
   std::vector<const uint32_t*> t;
   for (uint32_t i = 0; i < C; i++) t.push_back(coeffs + i * cs(n));
   for (uint32_t i = 0; i < C; i++) t.push_back(lde + i * cs(2 * n));
   for (uint32_t k = 0; k < 4; k++) t.push_back(comp + k * cs(2 * n));
   for (uint32_t k = 0; k < 4; k++) t.push_back(comp_lde + k * cs(4 * n));
+  const size_t n_job0 = t.size();
+  if (n_jobs > 1) {
+    auto all = [&](size_t first, size_t count) {
+      for (uint32_t j = 0; j < n_jobs; j++)
+        for (size_t i = 0; i < count; i++) t.push_back((const uint32_t*)((const char*)t[first + i] + j * jstride));
+    };
+    all(0, C);
+    all(C, C);
+    all(2 * C, 4);
+    all(2 * C + 4, 4);
+  }
   // one device region for every table of the job: pointer tables, the Merkle plans'
   // tables and the gather table, filled as one host image and uploaded once
   const uint32_t NL = L.n_line;
   std::vector<uint64_t> o_tree(3 + NL);
   uint64_t tb_bytes = align256(t.size() * sizeof(void*));
   o_tree[0] = tb_bytes;
-  tb_bytes += align256(merkle::MerklePlan::table_bytes(C, ln + 1));
+  tb_bytes += align256(merkle::MerklePlan::table_bytes(C, ln + 1, n_jobs));
   o_tree[1] = tb_bytes;
-  tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2));
+  tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2, n_jobs));
   for (uint32_t j = 1; j <= NL; j++) {
     o_tree[1 + j] = tb_bytes;
-    tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2 - j));
+    tb_bytes += align256(merkle::MerklePlan::table_bytes(4, ln + 2 - j, n_jobs));
   }
   const uint64_t o_ops = tb_bytes;
   tb_bytes += align256(sizeof(GatherOp) * gather_ops_upper_bound(p, L));
@@ -423,6 +466,18 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   t_lde = tb + C;
   t_comp = tb + 2 * C;
   t_comp_lde = tb + 2 * C + 4;
+  if (n_jobs > 1) {
+    const uint32_t* const* bt = tb + n_job0;
+    b_coeffs = bt;
+    b_lde = bt + (size_t)n_jobs * C;
+    b_comp = bt + (size_t)2 * n_jobs * C;
+    b_comp_lde = bt + (size_t)2 * n_jobs * C + 4 * n_jobs;
+  } else {
+    b_coeffs = t_coeffs;
+    b_lde = t_lde;
+    b_comp = t_comp;
+    b_comp_lde = t_comp_lde;
+  }
 
   // Merkle plans
   layers_t = tree_layers(base + A.tree_t, ln + 1);
@@ -430,10 +485,10 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   {
     std::vector<uint32_t> logs(C, ln + 1);
     tree_t = std::make_unique<merkle::MerklePlan>(C, t.data() + C, logs.data(), layers_t.data(), ln + 1,
-                                                  dtab + o_tree[0], image.data() + o_tree[0]);
+                                                  dtab + o_tree[0], image.data() + o_tree[0], n_jobs, jstride);
     std::vector<uint32_t> l4(4, ln + 2);
     tree_q = std::make_unique<merkle::MerklePlan>(4, t.data() + 2 * C + 4, l4.data(), layers_q.data(), ln + 2,
-                                                  dtab + o_tree[1], image.data() + o_tree[1]);
+                                                  dtab + o_tree[1], image.data() + o_tree[1], n_jobs, jstride);
   }
   for (uint32_t j = 1; j <= NL; j++) {
     const uint32_t m = ln + 2 - j;
@@ -442,7 +497,8 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
     for (int k = 0; k < 4; k++) cols[k] = line[j - 1] + k * cs(1ull << m);
     const uint32_t l4[4] = {m, m, m, m};
     tree_l.push_back(std::make_unique<merkle::MerklePlan>(4, cols, l4, layers_l.back().data(), m,
-                                                          dtab + o_tree[1 + j], image.data() + o_tree[1 + j]));
+                                                          dtab + o_tree[1 + j], image.data() + o_tree[1 + j],
+                                                          n_jobs, jstride));
   }
 
   // proof assembly table
@@ -496,7 +552,7 @@ Job::Job(const Params& p_, hipStream_t s) : p((validate(p_), p_)), L(p_), stream
   ops_ = (const GatherOp*)(dtab + o_ops);
   upload_to(tables_.p, image.data(), tb_bytes);
 
-  host_proof = take_pinned(4 * L.total);
+  host_proof = take_pinned(4 * L.total * n_jobs);
   hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
 }
 
@@ -512,43 +568,47 @@ Job::~Job() {
   arena = DevBuf();
 }
 
-void Job::enqueue(uint64_t seed) {
+void Job::enqueue(const uint64_t* seeds) {
   StreamScope scope(stream);
-  const uint32_t C = p.n_cols, ln = p.log_n;
-  const uint64_t n = 1ull << ln;
+  const uint32_t C = p.n_cols, ln = p.log_n, K = n_jobs;
+  const uint64_t n = 1ull << ln, js = jstride;
+  const dim3 one(1, 1, K);  // one block per job
 
-  timed("channel_begin", [&] { k_begin<<<1, 1, 0, stream>>>(jb, seed, p, L.n_line); });
-  wide_fib_trace_async((uint32_t* const*)t_coeffs, C, ln, &jb->seed);
-  fft::lde_same_size(t_coeffs, (uint32_t* const*)t_coeffs, (uint32_t* const*)t_lde, C, ln, 1, tw, itw, tree_len,
-                     fft::lde_variant_for(ln, 1, C));
+  Seeds sd{};
+  for (uint32_t k = 0; k < K; k++) sd.s[k] = seeds[k];
+  timed("channel_begin", [&] { k_begin<<<one, 1, 0, stream>>>(jb, sd, p, L.n_line, js); });
+  wide_fib_trace_async((uint32_t* const*)t_coeffs, C, ln, &jb->seed, K, js);
+  fft::lde_same_size(b_coeffs, (uint32_t* const*)b_coeffs, (uint32_t* const*)b_lde, K * C, ln, 1, tw, itw, tree_len,
+                     fft::lde_variant_for(ln, 1, (uint64_t)K * C));
   {
     const merkle::RootDraw rd{&jb->ch, &jb->alpha, 0};
     if (!tree_t->enqueue(nullptr, &rd))
-      timed("channel_trace_root", [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha); });
+      timed("channel_trace_root",
+            [&] { k_mix_root_draw<<<one, 1, 0, stream>>>(jb, layers_t[0], &jb->alpha, js); });
   }
 
-  qm31_powers_async(&jb->alpha, C - 2, powers);
-  vanishing_inverse_async(dinv, ln + 1, ln);
+  qm31_powers_async(&jb->alpha, C - 2, powers, K, js);
+  vanishing_inverse_async(dinv, ln + 1, ln, K, js);
   uint32_t* q4[4] = {comp, comp + cs(2 * n), comp + 2 * cs(2 * n), comp + 3 * cs(2 * n)};
-  wide_fib_quotients_async(q4, t_lde, powers, dinv, (uint32_t)(2 * n), C);
-  fft::lde_same_size(t_comp, (uint32_t* const*)t_comp, (uint32_t* const*)t_comp_lde, 4, ln + 1, 1, tw, itw, tree_len,
-                     fft::lde_variant_for(ln + 1, 1, 4));
+  wide_fib_quotients_async(q4, t_lde, powers, dinv, (uint32_t)(2 * n), C, K, js);
+  fft::lde_same_size(b_comp, (uint32_t* const*)b_comp, (uint32_t* const*)b_comp_lde, 4 * K, ln + 1, 1, tw, itw,
+                     tree_len, fft::lde_variant_for(ln + 1, 1, 4ull * K));
   {
     const merkle::RootDraw rd{&jb->ch, jb->z, 1};
     if (!tree_q->enqueue(nullptr, &rd))
-      timed("channel_oods_point", [&] { k_mix_root_draw_point<<<1, 1, 0, stream>>>(jb, layers_q[0]); });
+      timed("channel_oods_point", [&] { k_mix_root_draw_point<<<one, 1, 0, stream>>>(jb, layers_q[0], js); });
   }
 
-  eval_at_point_columns_async(t_coeffs, C, ln, jb->z, samples, eap_scratch);
-  eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch);
-  timed("channel_samples", [&] { k_mix_samples_draw<<<1, TREE_NT, 0, stream>>>(jb, samples, C + 4); });
+  eval_at_point_columns_async(t_coeffs, C, ln, jb->z, samples, eap_scratch, K, js);
+  eval_at_point_columns_async(t_comp, 4, ln + 1, jb->z, samples + C, eap_scratch, K, js);
+  timed("channel_samples", [&] { k_mix_samples_draw<<<one, TREE_NT, 0, stream>>>(jb, samples, C + 4, js); });
 
   uint32_t* f1c[4] = {f1, f1 + cs(2 * n), f1 + 2 * cs(2 * n), f1 + 3 * cs(2 * n)};
   uint32_t* f2c[4] = {f2, f2 + cs(4 * n), f2 + 2 * cs(4 * n), f2 + 3 * cs(4 * n)};
   accumulate_quotients_single_point_async(half_initial(ln + 1), half_step(ln + 1), ln + 1, t_lde, C, jb->z, samples,
-                                          &jb->beta, quot_scratch, f1c);
+                                          &jb->beta, quot_scratch, f1c, K, js);
   accumulate_quotients_single_point_async(half_initial(ln + 2), half_step(ln + 2), ln + 2, t_comp_lde, 4, jb->z,
-                                          samples + C, &jb->beta, quot_scratch, f2c);
+                                          samples + C, &jb->beta, quot_scratch, f2c, K, js);
 
   auto coords = [&](uint32_t* c, uint32_t m) {
     std::array<uint32_t*, 4> r;
@@ -557,31 +617,35 @@ void Job::enqueue(uint64_t seed) {
   };
   auto l1 = coords(line[0], ln + 1);
   fold_circle_into_line_async(itw, tree_len - (1ull << (ln + 1)), 4 * n, f2c, qzero(), l1.data(), &jb->fri_alpha[0],
-                              false);
+                              false, K, js);
   for (uint32_t j = 1; j <= L.n_line; j++) {
     const uint32_t m = ln + 2 - j;
     const merkle::RootDraw rd{&jb->ch, &jb->fri_alpha[j], 0};
     if (!tree_l[j - 1]->enqueue(nullptr, &rd))
       timed("channel_fri_root",
-            [&] { k_mix_root_draw<<<1, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j]); });
+            [&] { k_mix_root_draw<<<one, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j], js); });
     auto src = coords(line[j - 1], m), dst = coords(line[j], m - 1);
-    fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j]);
+    fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j], K,
+                    js);
     if (j == 1)
       fold_circle_into_line_async(itw, tree_len - (1ull << ln), 2 * n, f1c, qzero(), dst.data(), &jb->fri_alpha[1],
-                                  true);
+                                  true, K, js);
   }
   {
     const uint32_t nl = 1u << p.last_log;
     const uint32_t* last = line[L.n_line];
     Soa4c s{{last, last + cs(nl), last + 2 * cs(nl), last + 3 * cs(nl)}};
-    timed("channel_last_layer", [&] { k_mix_last<<<1, TREE_NT, 0, stream>>>(jb, s, nl); });
+    timed("channel_last_layer", [&] { k_mix_last<<<one, TREE_NT, 0, stream>>>(jb, s, nl, js); });
   }
-  grind_async(jb->ch.digest, p.pow_bits, &jb->nonce);
-  timed("channel_queries", [&] { k_mix_nonce_queries<<<1, 1, 0, stream>>>(jb, p.n_queries, ln + 2); });
+  grind_async(jb->ch.digest, p.pow_bits, &jb->nonce, K, js);
+  timed("channel_queries", [&] { k_mix_nonce_queries<<<one, 1, 0, stream>>>(jb, p.n_queries, ln + 2, js); });
   timed("proof_gather", [&] {
-    k_gather<<<ceil_div(n_ops, 256), 256, 0, stream>>>(ops_, n_ops, jb, proof);
+    k_gather<<<dim3(ceil_div(n_ops, 256), 1, K), 256, 0, stream>>>(ops_, n_ops, jb, proof, js);
   });
-  hip_check(hipMemcpyAsync(host_proof, proof, 4 * L.total, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+  // every job's proof words, one row per job
+  hip_check(hipMemcpy2DAsync(host_proof, 4 * L.total, proof, K > 1 ? js : 4 * L.total, 4 * L.total, K,
+                             hipMemcpyDeviceToHost, stream),
+            "hipMemcpy2DAsync D2H");
   hip_check(hipEventRecord(done, stream), "hipEventRecord");
 }
 

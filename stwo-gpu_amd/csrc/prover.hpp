@@ -15,7 +15,13 @@ namespace prover {
 
 struct Params {
   uint32_t n_cols, log_n, n_queries, pow_bits, last_log;
+  bool operator==(const Params& o) const {
+    return n_cols == o.n_cols && log_n == o.log_n && n_queries == o.n_queries && pow_bits == o.pow_bits &&
+           last_log == o.last_log;
+  }
 };
+// same-shape jobs proved together (every phase one launch for all of them)
+constexpr uint32_t MAX_BATCH = 16;
 void validate(const Params& p);
 
 // Word offsets of the proof (the layout the verifier parses).
@@ -28,13 +34,18 @@ struct Layout {
   explicit Layout(const Params& p);
 };
 
-uint64_t footprint_bytes(const Params& p);  // device bytes of one job
+uint64_t footprint_bytes(const Params& p, uint32_t n_jobs = 1);  // device bytes of a batch of n_jobs
 
+// A batch of n_jobs same-shape proofs (n_jobs = 1: one proof).  Job k's buffers sit
+// k * jstride bytes after job 0's in one arena; the pointers below are job 0's, and every
+// launch covers all jobs of the batch.
 struct Job {
   Params p;
   Layout L;
   hipStream_t stream;
-  DevBuf arena;  // every per-job device buffer
+  uint32_t n_jobs;
+  uint64_t jstride;
+  DevBuf arena;  // every per-job device buffer, n_jobs arenas of jstride bytes
   std::unique_ptr<merkle::MerklePlan> tree_t, tree_q;
   std::vector<std::unique_ptr<merkle::MerklePlan>> tree_l;
   DevBuf tables_;  // pointer tables, Merkle plan tables and the gather table
@@ -55,20 +66,26 @@ struct Job {
   qm31* samples;  // n_cols trace samples, then 4 composition samples
   void *eap_scratch, *quot_scratch;
   struct JobBlock* jb;
-  const uint32_t* const* t_coeffs;  // device pointer tables
+  const uint32_t* const* t_coeffs;  // device pointer tables (job 0)
   const uint32_t* const* t_lde;
   const uint32_t* const* t_comp;
   const uint32_t* const* t_comp_lde;
+  const uint32_t* const* b_coeffs;  // the same over every job of the batch (the LDE launches)
+  const uint32_t* const* b_lde;
+  const uint32_t* const* b_comp;
+  const uint32_t* const* b_comp_lde;
   std::vector<merkle::Hash*> layers_t, layers_q;
   std::vector<std::vector<merkle::Hash*>> layers_l;
   const uint32_t *tw, *itw;
   uint64_t tree_len;
 
   // Allocates everything and builds the plans; work is enqueued on `stream`.
-  Job(const Params& p, hipStream_t stream);
+  Job(const Params& p, hipStream_t stream, uint32_t n_jobs = 1);
   ~Job();
-  void enqueue(uint64_t seed);  // the whole proof, then the copy to host_proof and `done`
-  const uint32_t* result() const { return (const uint32_t*)host_proof; }
+  // the whole proof of every job (job k from seeds[k]), then the copy to host_proof and `done`
+  void enqueue(const uint64_t* seeds);
+  void enqueue(uint64_t seed) { enqueue(&seed); }
+  const uint32_t* result(uint32_t k = 0) const { return (const uint32_t*)host_proof + k * L.total; }
 };
 
 }  // namespace prover

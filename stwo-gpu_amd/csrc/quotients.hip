@@ -146,14 +146,24 @@ __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)
 
 constexpr int QG = 8;  // columns per load group (a multiple of the fold period 4)
 
+// js: byte stride between the jobs of a batched launch (blockIdx.z = job): the slot and batch
+// tables and the outputs move by the job's offset (the slots already hold that job's columns)
 template <int NB>
 __global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, uint32_t log_size,
                                                               const ColSlot<NB>* __restrict__ slots,
                                                               uint32_t n_slots, const Batch* __restrict__ batches,
                                                               uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
-                                                              uint32_t* __restrict__ o2, uint32_t* __restrict__ o3) {
+                                                              uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
+                                                              uint64_t js) {
   uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= (1u << log_size)) return;
+  const uint64_t jo = job_offset(js);
+  slots = jptr(slots, jo);
+  batches = jptr(batches, jo);
+  o0 = jptr(o0, jo);
+  o1 = jptr(o1, jo);
+  o2 = jptr(o2, jo);
+  o3 = jptr(o3, jo);
   uint64_t s[NB][4];
 #pragma unroll
   for (int b = 0; b < NB; b++) s[b][0] = s[b][1] = s[b][2] = s[b][3] = 0;
@@ -223,8 +233,15 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* con
                                                              const qm31* __restrict__ point,
                                                              const qm31* __restrict__ values, uint32_t count,
                                                              const qm31* __restrict__ rc_dev, ColSlot<1>* ss,
-                                                             uint32_t n_slots, Batch* bs) {
+                                                             uint32_t n_slots, Batch* bs, uint64_t js) {
   __shared__ qm31 sa[SB_NT], sb[SB_NT];
+  // batched jobs (blockIdx.z): the slots get the job's own column pointers
+  const uint64_t jo = job_offset(js);
+  point = jptr(point, jo);
+  values = jptr(values, jo);
+  rc_dev = jptr(rc_dev, jo);
+  ss = jptr(ss, jo);
+  bs = jptr(bs, jo);
   const qm31 px = point[0], py = point[1], rc = *rc_dev;
   const qm31 c = qsub(qm31{py.x, cneg(py.y)}, py);
   qm31 la = qzero(), lb = qzero();
@@ -235,9 +252,9 @@ __global__ __launch_bounds__(SB_NT) void single_batch_kernel(const uint32_t* con
     const qm31 bb = qsub(qmul(v, c), qmul(a, py));
     la = qadd(la, qmul(alpha, a));
     lb = qadd(lb, qmul(alpha, bb));
-    ss[j] = ColSlot<1>{cols[j], 1, 0, {alpha}};
+    ss[j] = ColSlot<1>{jptr(cols[j], jo), 1, 0, {alpha}};
   }
-  for (uint32_t j = count + threadIdx.x; j < n_slots; j += SB_NT) ss[j] = ColSlot<1>{cols[0], 0, 0, {qzero()}};
+  for (uint32_t j = count + threadIdx.x; j < n_slots; j += SB_NT) ss[j] = ColSlot<1>{jptr(cols[0], jo), 0, 0, {qzero()}};
   sa[threadIdx.x] = la;
   sb[threadIdx.x] = lb;
   __syncthreads();
@@ -311,22 +328,24 @@ static quot::DomainTables cached_domain_tables(uint32_t hi, uint32_t hs, uint32_
 
 // accumulate_quotients for one sample point shared by columns 0 .. n_cols-1 of d_cols, with
 // the point (x, y), the sample values and the random coefficient read from device memory.
+// n_jobs / js: same-shape jobs of a batched launch, js bytes apart (columns, point, values,
+// coefficient, scratch and outputs all move by the job's offset; the domain is shared).
 void accumulate_quotients_single_point_async(uint32_t hi, uint32_t hs, uint32_t log_size,
                                              const uint32_t* const* d_cols, uint32_t n_cols, const qm31* d_point,
                                              const qm31* d_values, const qm31* d_rc, void* d_scratch,
-                                             uint32_t* const out[4]) {
+                                             uint32_t* const out[4], uint32_t n_jobs, uint64_t js) {
   require(log_size >= 1 && log_size <= 30 && n_cols >= 1, "accumulate_quotients: bad sizes");
   const uint32_t n_slots = padded_slots(n_cols);
   quot::ColSlot<1>* ss = (quot::ColSlot<1>*)d_scratch;
   quot::Batch* bs = (quot::Batch*)((char*)d_scratch + slot_bytes(n_slots));
   const quot::DomainTables T = cached_domain_tables(hi, hs, log_size);
   timed("quotient_constants", [&] {
-    quot::single_batch_kernel<<<1, quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols, d_rc, ss,
-                                                                n_slots + quot::QG, bs);
+    quot::single_batch_kernel<<<dim3(1, 1, n_jobs), quot::SB_NT, 0, ctx().cur>>>(d_cols, d_point, d_values, n_cols,
+                                                                                d_rc, ss, n_slots + quot::QG, bs, js);
   });
   timed("accumulate_quotients", [&] {
-    quot::accumulate_cols_kernel<1><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        T, log_size, ss, n_slots, bs, out[0], out[1], out[2], out[3]);
+    quot::accumulate_cols_kernel<1><<<dim3(ceil_div(1ull << log_size, 256), 1, n_jobs), 256, 0, ctx().cur>>>(
+        T, log_size, ss, n_slots, bs, out[0], out[1], out[2], out[3], js);
   });
 }
 
@@ -369,7 +388,7 @@ void launch_cols_nb(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t*
   const quot::DomainTables T = cached_domain_tables(hi, hs, log_size);
   timed("accumulate_quotients", [&] {
     quot::accumulate_cols_kernel<NB><<<ceil_div(1ull << log_size, 256), 256, 0, ctx().cur>>>(
-        T, log_size, dsl, n_loop, dbs, out[0], out[1], out[2], out[3]);
+        T, log_size, dsl, n_loop, dbs, out[0], out[1], out[2], out[3], 0);
   });
 }
 void launch_cols(uint32_t hi, uint32_t hs, uint32_t log_size, const uint32_t* const* h_cols,

@@ -155,6 +155,15 @@ inline void timed(const char* name, F&& f) {
   launch_end(name);
 }
 
+// Several same-shape proof jobs in one launch: job z = blockIdx.z keeps its buffers z * js
+// bytes after job 0's (equal arena layouts), and tables hold job 0's pointers, so a kernel
+// adds the job's offset to every arena pointer it uses (js = 0 and one job elsewhere).
+template <class T>
+__device__ __forceinline__ T* jptr(T* p, uint64_t jo) {
+  return (T*)((char*)p + jo);
+}
+__device__ __forceinline__ uint64_t job_offset(uint64_t js) { return (uint64_t)blockIdx.z * js; }
+
 // Run `body` converting failures into a status; used at every ABI entry.
 template <class F>
 inline int guarded(F&& body) {

@@ -13,6 +13,12 @@
 // device-filling jobs from crowding each other while small ones still overlap.  A job
 // larger than the whole budget (or the work limit) is admitted alone.  Results are returned in
 // completion order with their submit / start / end times.
+//
+// Same-phase batching across jobs: with a batch limit set, admitting a small job (at most
+// max_batch_work trace elements) also takes up to max_batch - 1 queued jobs of the same
+// statement shape, and the group is proved as one batch (prover::Job with n_jobs > 1):
+// every phase of the proof is one launch covering all of its jobs, so a burst of small
+// proofs costs one proof's chain of launches instead of one chain per proof.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -37,7 +43,7 @@ struct Request {
   double t_submit;
 };
 struct Running {
-  Request r;
+  std::vector<Request> rs;  // the batch's requests (job k of the batch proves rs[k])
   std::unique_ptr<prover::Job> job;
   int slot;
   uint64_t bytes, work;  // device footprint, trace elements
@@ -58,6 +64,8 @@ struct Server {
   uint64_t budget;
   uint32_t max_inflight;
   std::atomic<uint64_t> work_limit{0};  // 0: none
+  std::atomic<uint32_t> max_batch{1};   // same-shape jobs proved together (1: no batching)
+  std::atomic<uint64_t> max_batch_work{0};  // only jobs of at most this many trace elements batch
   Clock::time_point t0 = Clock::now();
   std::vector<hipStream_t> streams;
   std::vector<bool> slot_busy;
@@ -76,16 +84,21 @@ struct Server {
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
   void finish(Running& r, int status) {
-    Result res{r.r.id, status, {}, r.r.t_submit, r.t_start, now(), r.host_s,
-               status == STWO_HIP_OK ? std::string() : std::string("job failed on the device")};
-    if (status == STWO_HIP_OK) {
-      const uint32_t* pr = r.job->result();
-      res.proof.assign(pr, pr + r.job->L.total);
-      // no proof-of-work nonce found within the search bound
-      if (pr[r.job->L.nonce] == 0xffffffffu && pr[r.job->L.nonce + 1] == 0xffffffffu) {
-        res.status = STWO_HIP_ERR_ARG;
-        res.error = "no proof-of-work nonce within the search bound";
+    const double t_end = now();
+    std::vector<Result> out;
+    for (size_t k = 0; k < r.rs.size(); k++) {
+      Result res{r.rs[k].id, status, {}, r.rs[k].t_submit, r.t_start, t_end, r.host_s,
+                 status == STWO_HIP_OK ? std::string() : std::string("job failed on the device")};
+      if (status == STWO_HIP_OK) {
+        const uint32_t* pr = r.job->result((uint32_t)k);
+        res.proof.assign(pr, pr + r.job->L.total);
+        // no proof-of-work nonce found within the search bound
+        if (pr[r.job->L.nonce] == 0xffffffffu && pr[r.job->L.nonce + 1] == 0xffffffffu) {
+          res.status = STWO_HIP_ERR_ARG;
+          res.error = "no proof-of-work nonce within the search bound";
+        }
       }
+      out.push_back(std::move(res));
     }
     {
       auto g = lock();
@@ -95,7 +108,7 @@ struct Server {
     inflight_bytes -= r.bytes;
     inflight_work -= r.work;
     std::lock_guard<std::mutex> l(m);
-    results.push_back(std::move(res));
+    for (auto& res : out) results.push_back(std::move(res));
     cv_done.notify_all();
   }
 
@@ -115,25 +128,47 @@ struct Server {
       }
       // admissions, in arrival order
       for (;;) {
-        Request r;
+        std::vector<Request> rs;
+        uint64_t need = 0, w = 0;
         {
           std::lock_guard<std::mutex> l(m);
           if (queue.empty() || running.size() >= max_inflight) break;
-          const uint64_t need = prover::footprint_bytes(queue.front().p);
-          const uint64_t w = job_work(queue.front().p), wl = work_limit.load();
-          if (!running.empty() && (inflight_bytes + need > budget || (wl && inflight_work + w > wl))) break;
-          r = queue.front();
+          const prover::Params p = queue.front().p;
+          // the head job, and (small jobs, batching on) queued jobs of the same shape with it
+          uint32_t k = 1;
+          const uint32_t mb = max_batch.load();
+          if (mb > 1 && job_work(p) <= max_batch_work.load())
+            for (size_t i = 1; i < queue.size() && k < mb; i++) k += queue[i].p == p;
+          const uint64_t wl = work_limit.load();
+          // a batch that does not fit next to the jobs in flight shrinks before it waits
+          for (; k >= 1; k--) {
+            need = prover::footprint_bytes(p, k);
+            w = job_work(p) * k;
+            if (running.empty() || (inflight_bytes + need <= budget && (!wl || inflight_work + w <= wl))) break;
+          }
+          if (k == 0) break;
+          rs.push_back(queue.front());
           queue.pop_front();
+          for (size_t i = 0; i < queue.size() && rs.size() < k;) {
+            if (queue[i].p == p) {
+              rs.push_back(queue[i]);
+              queue.erase(queue.begin() + i);
+            } else {
+              i++;
+            }
+          }
         }
         int slot = 0;
         while (slot_busy[slot]) slot++;
-        Running run{r, nullptr, slot, prover::footprint_bytes(r.p), job_work(r.p), now(), 0.0};
+        Running run{rs, nullptr, slot, need, w, now(), 0.0};
         int st = STWO_HIP_OK;
         {
           auto g = lock();
           try {
-            run.job = std::make_unique<prover::Job>(r.p, streams[slot]);
-            run.job->enqueue(r.seed);
+            std::vector<uint64_t> seeds;
+            for (const Request& q : rs) seeds.push_back(q.seed);
+            run.job = std::make_unique<prover::Job>(rs[0].p, streams[slot], (uint32_t)rs.size());
+            run.job->enqueue(seeds.data());
             run.host_s = now() - run.t_start;
           } catch (const Failure& f) {
             st = f.status;
@@ -142,7 +177,8 @@ struct Server {
         }
         if (st != STWO_HIP_OK) {
           std::lock_guard<std::mutex> l(m);
-          results.push_back(Result{r.id, st, {}, r.t_submit, run.t_start, now(), 0.0, stwo_hip_last_error()});
+          for (const Request& q : rs)
+            results.push_back(Result{q.id, st, {}, q.t_submit, run.t_start, now(), 0.0, stwo_hip_last_error()});
           cv_done.notify_all();
           continue;
         }
@@ -205,6 +241,28 @@ extern "C" int stwo_hip_wide_fib_prove(const StwoWideFibParams* params, uint64_t
   });
 }
 
+extern "C" int stwo_hip_wide_fib_prove_batch(const StwoWideFibParams* params, const uint64_t* seeds, uint32_t n_jobs,
+                                             uint32_t* proofs, uint64_t proof_capacity_words) {
+  return guarded([&] {
+    prover::Params p = params_of(params);
+    require(seeds != nullptr && n_jobs >= 1 && n_jobs <= prover::MAX_BATCH,
+            "wide fibonacci prove batch: 1 .. 16 jobs with their seeds");
+    const uint64_t words = prover::Layout(p).total;
+    require(proofs != nullptr && proof_capacity_words >= words * n_jobs,
+            "wide fibonacci prove batch: proof buffer too small");
+    prover::Job job(p, ctx().stream, n_jobs);
+    job.enqueue(seeds);
+    hip_check(hipEventSynchronize(job.done), "hipEventSynchronize");
+    sync();
+    for (uint32_t k = 0; k < n_jobs; k++) {
+      const uint32_t* r = job.result(k);
+      require(!(r[job.L.nonce] == 0xffffffffu && r[job.L.nonce + 1] == 0xffffffffu),
+              "wide fibonacci prove batch: no proof-of-work nonce within the search bound");
+      std::copy(r, r + words, proofs + k * words);
+    }
+  });
+}
+
 extern "C" int stwo_hip_server_create(StwoProofServer** out, uint64_t hbm_budget_bytes, uint32_t max_inflight) {
   return guarded([&] {
     require(out != nullptr && max_inflight >= 1 && max_inflight <= 16, "server: max_inflight must be in [1, 16]");
@@ -227,6 +285,13 @@ extern "C" int stwo_hip_server_create(StwoProofServer** out, uint64_t hbm_budget
 extern "C" int stwo_hip_server_limit_work(StwoProofServer* h, uint64_t max_trace_elements) {
   if (!h) return STWO_HIP_ERR_ARG;
   h->s.work_limit.store(max_trace_elements);
+  return STWO_HIP_OK;
+}
+
+extern "C" int stwo_hip_server_batch(StwoProofServer* h, uint32_t max_batch, uint64_t max_batch_work) {
+  if (!h || max_batch < 1 || max_batch > prover::MAX_BATCH) return STWO_HIP_ERR_ARG;
+  h->s.max_batch.store(max_batch);
+  h->s.max_batch_work.store(max_batch_work);
   return STWO_HIP_OK;
 }
 

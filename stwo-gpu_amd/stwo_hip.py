@@ -140,6 +140,8 @@ def _declare(L):
         "stwo_hip_fri_fold_chain_async": (C.c_int, [vp, u64, u32, vp, u32, vp, vp]),
         "stwo_hip_eval_at_points_columns_noalloc": (C.c_int, [vp, u32, u32, vp, u32, vp]),
         "stwo_hip_merkle_auth_paths": (C.c_int, [vp, u32, u32p, u32, u32p]),
+        "stwo_hip_server_batch": (C.c_int, [vp, u32, u64]),
+        "stwo_hip_wide_fib_prove_batch": (C.c_int, [C.POINTER(WideFibParams), vp, u32, u32p, u64]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),
         "stwo_hip_server_wait": (C.c_int, [vp, u32, C.POINTER(u64), C.POINTER(C.c_int), u32p, u64,
                                            C.POINTER(u64), C.POINTER(C.c_double)]),
@@ -816,15 +818,28 @@ def wide_fib_prove(params, seed):
     return out
 
 
-class ProofServer:
-    """Continuous-batching proof server (stwo_hip_server_*)."""
+def wide_fib_prove_batch(params, seeds):
+    """Proofs of one statement shape for several seeds, proved together (one launch per phase)."""
+    words, _ = wide_fib_proof_size(params)
+    sd = np.ascontiguousarray(seeds, np.uint64)
+    out = np.zeros(words * sd.size, np.uint32)
+    _c(lib().stwo_hip_wide_fib_prove_batch(C.byref(params), sd.ctypes.data_as(vp), sd.size, out.ctypes.data_as(u32p),
+                                           out.size))
+    return out.reshape(sd.size, words)
 
-    def __init__(self, hbm_budget_bytes, max_inflight=4, work_limit=0):
+
+class ProofServer:
+    """Continuous-batching proof server (stwo_hip_server_*).  max_batch > 1: same-shape jobs of at
+    most max_batch_work trace elements are proved together as batches."""
+
+    def __init__(self, hbm_budget_bytes, max_inflight=4, work_limit=0, max_batch=1, max_batch_work=0):
         h = vp()
         _c(lib().stwo_hip_server_create(C.byref(h), hbm_budget_bytes, max_inflight))
         self.handle = h.value
         if work_limit:
             _c(lib().stwo_hip_server_limit_work(self.handle, work_limit))
+        if max_batch > 1:
+            _c(lib().stwo_hip_server_batch(self.handle, max_batch, max_batch_work))
 
     def submit(self, params, seed):
         jid = u64()

@@ -41,6 +41,59 @@ def test_large_proof_verifies():
     assert (S.wide_fib_prove(p, 99) == w).all()
 
 
+def test_largest_serving_proof_verifies():
+    # the top of the serving range: 512 columns of 2^20 rows
+    p = S.wide_fib_params(512, 20, 3, 10, 2)
+    info = V.verify(S.wide_fib_prove(p, 4242))
+    assert info["n_cols"] == 512
+
+
+@pytest.mark.parametrize("C,log_n,Q,pow_bits,last_log,k", [
+    (3, 4, 1, 0, 2, 2), (16, 7, 3, 6, 2, 5), (40, 6, 3, 6, 3, 16), (64, 12, 3, 10, 2, 8), (100, 14, 3, 10, 3, 3),
+    (32, 15, 3, 8, 2, 4)])
+def test_batch_proofs_match_alone(C, log_n, Q, pow_bits, last_log, k):
+    # same-phase batching: every launch covers the k jobs; each proof equals the job's proof alone
+    p = S.wide_fib_params(C, log_n, Q, pow_bits, last_log)
+    seeds = [7000 + 31 * i for i in range(k)]
+    got = S.wide_fib_prove_batch(p, seeds)
+    for i, seed in enumerate(seeds):
+        alone = S.wide_fib_prove(p, seed)
+        diff = np.nonzero(got[i] != alone)[0]
+        assert diff.size == 0, f"job {i}: first differing word {diff[:8]}"
+    assert not (got[0] == got[-1]).all() or k == 1
+    V.verify(got[-1])
+
+
+def test_batch_proof_matches_oracle():
+    p = S.wide_fib_params(17, 8, 5, 8, 4)
+    got = S.wide_fib_prove_batch(p, [3, 4, 5])
+    for i, seed in enumerate([3, 4, 5]):
+        assert (got[i] == OP.prove(17, 8, 5, 8, 4, seed)).all(), f"job {i}"
+
+
+def test_server_batches_same_shape_jobs():
+    # a burst of small jobs of three shapes: batched admission proves same-shape jobs together,
+    # and every result is the job's proof alone
+    shapes = [(32, 12), (64, 13), (8, 14)]
+    jobs = [(S.wide_fib_params(C, ln, 3, 8, 2), 2000 + i) for i in range(24) for C, ln in [shapes[i % 3]]]
+    alone = {seed: S.wide_fib_prove(p, seed) for p, seed in jobs}
+    srv = S.ProofServer(16 << 30, 4, 0, max_batch=8, max_batch_work=1 << 24)
+    ids = {srv.submit(p, seed): seed for p, seed in jobs}
+    got, spans = {}, {}
+    while len(got) < len(jobs):
+        r = srv.wait(5_000_000)
+        assert r is not None, "server stalled"
+        jid, st, proof, times = r
+        assert st == 0
+        got[ids[jid]] = proof
+        spans[ids[jid]] = (times[1], times[2])
+    srv.close()
+    for seed, proof in got.items():
+        assert (proof == alone[seed]).all(), f"job with seed {seed} differs from its proof alone"
+    # some jobs were proved in one batch: they share their start and end times
+    assert len(set(spans.values())) < len(jobs)
+
+
 @pytest.mark.parametrize("bits", [0, 1, 8, 14])
 def test_grind_smallest_nonce(bits):
     digest = np.arange(8, dtype=np.uint32) * 0x01010101 + bits
