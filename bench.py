@@ -569,19 +569,19 @@ def bench_batched(args):
                   "algorithmic_bytes_per_launch": pipe, "avg_launch_ms": round(ms_graph, 4)}, extra)
 
 
-def serving_jobs(n_jobs, rate, seed=0x5EED0005):
+def serving_jobs(n_jobs, rate, seed=0x5EED0005, logs=(12, 21), cols=(32, 64, 128, 256, 512)):
     """Poisson arrivals (seeded): (arrival_s, n_cols, log_n, trace seed)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
-    cols = rng.choice([32, 64, 128, 256, 512], n_jobs)
-    logs = rng.integers(12, 21, n_jobs)
-    return [(float(t[i]), int(cols[i]), int(logs[i]), 0xA000 + i) for i in range(n_jobs)]
+    c = rng.choice(list(cols), n_jobs)
+    lg = rng.integers(logs[0], logs[1], n_jobs)
+    return [(float(t[i]), int(c[i]), int(lg[i]), 0xA000 + i) for i in range(n_jobs)]
 
 
-def _serve(S, jobs, budget, slots, pow_bits, last_log, work_limit=0):
+def _serve(S, jobs, budget, slots, pow_bits, last_log, work_limit=0, max_batch=1, max_batch_work=0):
     """Submit each job at its arrival time; returns per-job ((submit, start, end), proof words)."""
-    srv = S.ProofServer(budget, slots, work_limit)
+    srv = S.ProofServer(budget, slots, work_limit, max_batch, max_batch_work)
     t0 = time.perf_counter()
     done, ids = {}, {}
     i = 0
@@ -624,20 +624,25 @@ def _serve_plain(jobs, pow_bits, last_log):
 
 def bench_serving(args):
     """Wide-Fibonacci proof serving: a seeded Poisson stream of prove jobs (n_cols in
-    {32..512}, log_n in {12..20}, blowup 2, 3 queries) through the continuous-batching
-    server, against the same stream proved one job at a time by the same device prover
-    and by the reference-shaped plain path (one synchronous call per backend operation).
-    At the default arrival rate jobs arrive faster than one GPU proves them, so proofs/s
-    is capacity; the sweep rates show latency under partial load."""
+    {32..512}, log_n in {12..20}, blowup 2, 3 queries) through three modes on one GPU:
+      * batched: the continuous-batching server (8 slot streams, HBM admission, a work limit,
+        same-phase batching of same-shape small jobs);
+      * device prover one job at a time (the server with one slot, no batching);
+      * plain: the reference-shaped path (one synchronous part-1 ABI call per backend operation,
+        host channel, one gather per tree for the decommitments).
+    Capacity: the burst stream (offered load above what one GPU proves).  Equal p99: every mode
+    runs the same job stream at rising Poisson rates; the latency target is 1.5x the plain
+    path's own p99 at the lowest rate, and each mode is credited with the proofs/s of the
+    highest rate whose p99 stays within it.  A burst of small same-shape jobs (log_n 12..14)
+    shows what the same-phase batching adds over overlapping streams."""
     import numpy as np
     import stwo_hip as S
 
     n_jobs, pow_bits, last_log = args.serve_jobs, 10, 2
     budget = args.serve_budget_gib << 30
-    rate = args.serve_rate
-    jobs = serving_jobs(n_jobs, rate)
+    wl, mb, mbw = args.serve_work_limit, args.serve_max_batch, args.serve_batch_work
     # warm the twiddle cache and the memory pool with one pass of every shape, off the clock
-    shapes = sorted({(C, ln) for _, C, ln, _ in jobs})
+    shapes = sorted({(C, ln) for _, C, ln, _ in serving_jobs(n_jobs, 1000.0)})
     warm = S.ProofServer(budget, args.serve_slots)
     for C, ln in shapes:
         warm.submit(S.wide_fib_params(C, ln, 3, pow_bits, last_log), 1)
@@ -655,25 +660,62 @@ def bench_serving(args):
                 "p99_ms": round(float(np.percentile(lat, 99) * 1e3), 2), "makespan_s": round(last - first, 4),
                 "host_ms_per_job": round(float(np.mean([done[i][0][3] for i in range(n)])) * 1e3, 3)}
 
-    wl = args.serve_work_limit
-    batched = summarize(_serve(S, jobs, budget, args.serve_slots, pow_bits, last_log, wl), n_jobs)
-    one_slot = summarize(_serve(S, jobs, budget, 1, pow_bits, last_log), n_jobs)
-    plain = summarize(_serve_plain(jobs, pow_bits, last_log), n_jobs)
-    sweep = []
-    for r in (args.serve_sweep or []):
-        js = serving_jobs(n_jobs, r)
-        sweep.append({"rate": r,
-                      "batched": summarize(_serve(S, js, budget, args.serve_slots, pow_bits, last_log, wl), n_jobs),
-                      "device_prover_one_at_a_time": summarize(_serve(S, js, budget, 1, pow_bits, last_log), n_jobs)})
-    elems = sum(C << ln for _, C, ln, _ in jobs)
-    extra = {"batched": batched, "plain_reference_shaped": plain, "device_prover_one_at_a_time": one_slot,
-             "throughput_ratio_vs_plain": round(batched["proofs_per_s"] / plain["proofs_per_s"], 2),
-             "throughput_ratio_vs_one_at_a_time": round(batched["proofs_per_s"] / one_slot["proofs_per_s"], 2),
+    modes = {
+        "batched": lambda js: _serve(S, js, budget, args.serve_slots, pow_bits, last_log, wl, mb, mbw),
+        "device_prover_one_at_a_time": lambda js: _serve(S, js, budget, 1, pow_bits, last_log),
+        "plain_reference_shaped": lambda js: _serve_plain(js, pow_bits, last_log),
+    }
+    rate = args.serve_rate
+    capacity = {m: summarize(f(serving_jobs(n_jobs, rate)), n_jobs) for m, f in modes.items()}
+
+    # equal p99: rising rates, each mode until its p99 leaves the target
+    rates = sorted(args.serve_sweep)
+    slo_ms, sweep, best = None, {m: [] for m in modes}, {}
+    for m in ["plain_reference_shaped", "device_prover_one_at_a_time", "batched"]:
+        for r in rates:
+            res = summarize(modes[m](serving_jobs(n_jobs, r)), n_jobs)
+            res["rate"] = r
+            sweep[m].append(res)
+            if slo_ms is None:
+                slo_ms = round(1.5 * res["p99_ms"], 2)
+            if res["p99_ms"] > slo_ms:
+                break
+            best[m] = res
+    at_slo = {m: best.get(m) for m in modes}
+
+    def ratio(a, b):
+        return round(at_slo[a]["proofs_per_s"] / at_slo[b]["proofs_per_s"], 2) if at_slo[a] and at_slo[b] else None
+
+    # small same-shape jobs in a burst (all arrive within ~1 ms, so arrivals do not pace them):
+    # streams only vs streams + same-phase batching
+    small = serving_jobs(n_jobs, 1e5, seed=0x5EED0006, logs=(12, 15), cols=(32, 64))
+    small_res = {
+        "batched_same_phase": summarize(_serve(S, small, budget, args.serve_slots, pow_bits, last_log, wl, mb, mbw),
+                                        n_jobs),
+        "streams_only": summarize(_serve(S, small, budget, args.serve_slots, pow_bits, last_log, wl), n_jobs),
+        "device_prover_one_at_a_time": summarize(_serve(S, small, budget, 1, pow_bits, last_log), n_jobs),
+    }
+    elems = sum(C << ln for _, C, ln, _ in serving_jobs(n_jobs, rate))
+    extra = {"capacity": capacity,
+             "throughput_ratio_vs_plain": round(capacity["batched"]["proofs_per_s"] /
+                                                capacity["plain_reference_shaped"]["proofs_per_s"], 2),
+             "throughput_ratio_vs_one_at_a_time": round(capacity["batched"]["proofs_per_s"] /
+                                                        capacity["device_prover_one_at_a_time"]["proofs_per_s"], 2),
+             "equal_p99": {"p99_target_ms": slo_ms, "at_target": at_slo,
+                           "ratio_batched_vs_plain": ratio("batched", "plain_reference_shaped"),
+                           "ratio_batched_vs_one_at_a_time": ratio("batched", "device_prover_one_at_a_time"),
+                           "sweep": sweep},
+             "small_same_shape_burst": {"workload": f"{n_jobs} jobs arriving at 1e5/s, n_cols in {{32, 64}}, "
+                                                    "log_n in 12..14",
+                                        **small_res,
+                                        "ratio_batching_vs_streams": round(
+                                            small_res["batched_same_phase"]["proofs_per_s"] /
+                                            small_res["streams_only"]["proofs_per_s"], 2)},
              "jobs": n_jobs, "arrival_rate_per_s": rate, "slots": args.serve_slots, "work_limit": wl,
-             "hbm_budget_gib": args.serve_budget_gib, "trace_elements_total": elems,
-             "pow_bits": pow_bits, "last_layer_log": last_log, "rate_sweep": sweep}
-    return _line("served wide-Fibonacci proofs/s (continuous batching)", batched["proofs_per_s"], "proofs/s",
-                 1e3 / batched["proofs_per_s"], args,
+             "max_batch": mb, "max_batch_work": mbw, "hbm_budget_gib": args.serve_budget_gib,
+             "trace_elements_total": elems, "pow_bits": pow_bits, "last_layer_log": last_log}
+    return _line("served wide-Fibonacci proofs/s (continuous batching)", capacity["batched"]["proofs_per_s"],
+                 "proofs/s", 1e3 / capacity["batched"]["proofs_per_s"], args,
                  {"workload": f"serving: {n_jobs} Poisson(rate {rate}/s) wide-Fibonacci prove jobs, n_cols in "
                               "{32,64,128,256,512}, log_n in 12..20, blowup 2, 3 queries, Blake2s channel"},
                  extra=extra)
@@ -700,8 +742,12 @@ def main():
                     help="trace elements (n_cols x 2^log_n) in flight at most (0: no limit); 2^31, four of "
                          "the largest jobs, measured best with 8 slots")
     ap.add_argument("--serve-budget-gib", type=int, default=96)
-    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[400.0, 1000.0],
-                    help="extra arrival rates to run both device modes at (latency under partial load)")
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[25.0, 50.0, 100.0, 200.0, 400.0, 800.0, 1600.0],
+                    help="arrival rates (jobs/s) of the equal-p99 comparison")
+    ap.add_argument("--serve-max-batch", type=int, default=8,
+                    help="same-shape jobs proved as one batch by the server (1: no same-phase batching)")
+    ap.add_argument("--serve-batch-work", type=int, default=1 << 22,
+                    help="only jobs of at most this many trace elements (n_cols x 2^log_n) batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()

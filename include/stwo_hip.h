@@ -347,6 +347,10 @@ int stwo_hip_eval_at_points_columns_noalloc(const uint32_t *const *columns, uint
  * BLAKE2s-256(digest || lo32(nonce) || hi32(nonce)) starts (as a little-endian 64-bit
  * value) with pow_bits zero bits.  digest: 8 host words. */
 int stwo_hip_grind(const uint32_t *digest, uint32_t pow_bits, uint64_t *nonce);
+/* The channel's tree hash of n_words words (16-word leaves F0(block), pairwise levels, an
+ * unpaired last node carried up; F0 = BLAKE2s compression from the zero state), computed on
+ * the HOST: for a Fiat-Shamir channel kept on the CPU, as stwo keeps its channel.  out: 8 words. */
+int stwo_hip_host_tree_hash(const uint32_t *words, uint64_t n_words, uint32_t *out);
 
 /* Wide-Fibonacci prover (the WideFibComponent of stwo_gpu_backend/src/examples/wide_fibonacci.rs
  * proved end to end on the device: trace LDE + commitment, composition, OODS samples, DEEP

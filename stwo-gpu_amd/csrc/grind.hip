@@ -8,6 +8,9 @@
 // smallest hit is always tested (every hit recorded before it is larger) and the
 // result does not depend on scheduling.  Every lane also stops after max_rounds, so
 // the launch always drains; no hit within the bound is reported as an error.
+#include <array>
+#include <vector>
+
 #include "channel.hpp"
 #include "runtime.hpp"
 
@@ -58,6 +61,37 @@ void grind_async(const uint32_t* d_digest, uint32_t pow_bits, uint64_t* d_nonce,
 }  // namespace sh
 
 using namespace sh;
+
+// The channel's tree hash of a word sequence (blake2s.hpp), on the host: for a Fiat-Shamir
+// channel kept on the CPU next to the device backend, as stwo's is.
+extern "C" int stwo_hip_host_tree_hash(const uint32_t* words, uint64_t n_words, uint32_t* out) {
+  if (!out || (n_words && !words)) return STWO_HIP_ERR_ARG;
+  std::vector<std::array<uint32_t, 8>> level((std::max<uint64_t>(n_words, 1) + 15) / 16);
+  for (uint64_t i = 0; i < level.size(); i++) {
+    uint32_t m[16] = {0};
+    for (uint64_t j = 0; j < 16 && 16 * i + j < n_words; j++) m[j] = words[16 * i + j];
+    uint32_t h[8];
+    b2s::f0(m, h);
+    std::copy(h, h + 8, level[i].begin());
+  }
+  while (level.size() > 1) {
+    std::vector<std::array<uint32_t, 8>> up((level.size() + 1) / 2);
+    for (size_t i = 0; i < up.size(); i++) {
+      if (2 * i + 1 < level.size()) {
+        uint32_t m[16], h[8];
+        std::copy(level[2 * i].begin(), level[2 * i].end(), m);
+        std::copy(level[2 * i + 1].begin(), level[2 * i + 1].end(), m + 8);
+        b2s::f0(m, h);
+        std::copy(h, h + 8, up[i].begin());
+      } else {
+        up[i] = level[2 * i];
+      }
+    }
+    level.swap(up);
+  }
+  std::copy(level[0].begin(), level[0].end(), out);
+  return STWO_HIP_OK;
+}
 
 extern "C" int stwo_hip_grind(const uint32_t* digest, uint32_t pow_bits, uint64_t* nonce) {
   return guarded([&] {

@@ -140,6 +140,7 @@ def _declare(L):
         "stwo_hip_fri_fold_chain_async": (C.c_int, [vp, u64, u32, vp, u32, vp, vp]),
         "stwo_hip_eval_at_points_columns_noalloc": (C.c_int, [vp, u32, u32, vp, u32, vp]),
         "stwo_hip_merkle_auth_paths": (C.c_int, [vp, u32, u32p, u32, u32p]),
+        "stwo_hip_host_tree_hash": (C.c_int, [u32p, u64, u32p]),
         "stwo_hip_server_batch": (C.c_int, [vp, u32, u64]),
         "stwo_hip_wide_fib_prove_batch": (C.c_int, [C.POINTER(WideFibParams), vp, u32, u32p, u64]),
         "stwo_hip_server_submit": (C.c_int, [vp, C.POINTER(WideFibParams), u64, C.POINTER(u64)]),

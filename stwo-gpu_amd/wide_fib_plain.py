@@ -1,16 +1,19 @@
 """Plain per-job wide-Fibonacci prover: the same proof as stwo_hip_wide_fib_prove, driven the
 way stwo's prover drives the reference CudaBackend -- one synchronous call of the
-reference-shaped ABI (include/stwo_hip.h part 1) per backend operation, Fiat-Shamir and
-proof of work on the host, values for the decommitments copied back per row.
+reference-shaped ABI (include/stwo_hip.h part 1) per backend operation, the Fiat-Shamir
+channel on the host (hashlib BLAKE2s and the library's host tree hash, in place of stwo's
+Rust channel), the decommitted values and authentication paths gathered by one call per
+tree.
 
 Per operation it follows the trait implementations of the reference crate:
 interpolate_columns (poly.rs:57-82), extend + evaluate_polynomials (poly.rs:96-168),
 MerkleOps::commit_on_layer per layer (blake2s.rs:19-97), the wide-Fibonacci composition
 (examples/wide_fibonacci.rs:90-141), eval_at_point per column (poly.rs:84-94),
 accumulate_quotients (quotient.rs:17-115), fold_circle_into_line / fold_line
-(fri.rs:15-66); grind runs on the CPU as the reference's GrindOps does
-(backend.rs:14-18).  The trace and the vanishing-polynomial inverses are generated on
-the device (extension entry points) so both paths prove the same statement.
+(fri.rs:15-66).  Proof of work uses the library's grind (the reference delegates it to
+the CPU SimdBackend, backend.rs:14-18; a Python nonce loop would measure the
+interpreter).  The trace and the vanishing-polynomial inverses are generated on the device
+(extension entry points) so both paths prove the same statement.
 
 It is the baseline the proof server is measured against, and a second, independent
 assembly of the protocol of csrc/prover.hip: its proofs must equal the device prover's
@@ -84,60 +87,46 @@ class Channel:
         return out
 
 
-_SIGMA = [
-    (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15), (14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3),
-    (11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4), (7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8),
-    (9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13), (2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9),
-    (12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11), (13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10),
-    (6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5), (10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0)]
-_IV = (0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C, 0x1F83D9AB, 0x5BE0CD19)
-_M32 = 0xFFFFFFFF
-
-
-def _f0(m):
-    """BLAKE2s compression from the zero state, zero counter and flags (RFC 7693)."""
-    v = [0] * 8 + list(_IV)
-
-    def rot(x, c):
-        return ((x >> c) | (x << (32 - c))) & _M32
-
-    def g(a, b, c, d, x, y):
-        v[a] = (v[a] + v[b] + x) & _M32
-        v[d] = rot(v[d] ^ v[a], 16)
-        v[c] = (v[c] + v[d]) & _M32
-        v[b] = rot(v[b] ^ v[c], 12)
-        v[a] = (v[a] + v[b] + y) & _M32
-        v[d] = rot(v[d] ^ v[a], 8)
-        v[c] = (v[c] + v[d]) & _M32
-        v[b] = rot(v[b] ^ v[c], 7)
-    for s in _SIGMA:
-        g(0, 4, 8, 12, m[s[0]], m[s[1]])
-        g(1, 5, 9, 13, m[s[2]], m[s[3]])
-        g(2, 6, 10, 14, m[s[4]], m[s[5]])
-        g(3, 7, 11, 15, m[s[6]], m[s[7]])
-        g(0, 5, 10, 15, m[s[8]], m[s[9]])
-        g(1, 6, 11, 12, m[s[10]], m[s[11]])
-        g(2, 7, 8, 13, m[s[12]], m[s[13]])
-        g(3, 4, 9, 14, m[s[14]], m[s[15]])
-    return [v[i] ^ v[i + 8] for i in range(8)]
-
-
 def tree_root(words):
-    """Tree hash of a word sequence (csrc/blake2s.hpp): zero-padded 16-word leaves, pairwise
-    levels, an unpaired last node carried up."""
-    nodes = []
-    for i in range(0, len(words), 16):
-        blk = [int(w) for w in words[i:i + 16]]
-        nodes.append(_f0(blk + [0] * (16 - len(blk))))
-    while len(nodes) > 1:
-        nodes = [_f0(nodes[i] + nodes[i + 1]) if i + 1 < len(nodes) else nodes[i] for i in range(0, len(nodes), 2)]
-    return nodes[0]
+    """Tree hash of a word sequence (csrc/blake2s.hpp), computed on the host by the library."""
+    w = np.ascontiguousarray(np.array(words, dtype=np.uint64).astype(np.uint32))
+    out = np.zeros(8, np.uint32)
+    S.check(S.lib().stwo_hip_host_tree_hash(w.ctypes.data_as(S.u32p), w.size, out.ctypes.data_as(S.u32p)))
+    return [int(x) for x in out]
 
 
-def pow_ok(digest, nonce, bits):
-    h = _b2s(list(digest) + [nonce & 0xFFFFFFFF, nonce >> 32])
-    v = h[0] | (h[1] << 32)
-    return bits == 0 or (v & ((1 << bits) - 1)) == 0
+# ------------------------------------------------------------------ QM31 powers, vectorised
+def _m31_mul(a, b):
+    return (a.astype(np.uint64) * b.astype(np.uint64)) % P
+
+
+def _qm_mul_vec(x, y):
+    """Row-wise QM31 products of (n, 4) uint64 arrays: (a + b i) + (c + d i) u, u^2 = 2 + i."""
+    a, b, c, d = (x[:, k] for k in range(4))
+    e, f, g, h = (y[:, k] for k in range(4))
+    re0 = (_m31_mul(a, e) + P * P - _m31_mul(b, f)) % P  # (a + bi)(e + fi)
+    im0 = (_m31_mul(a, f) + _m31_mul(b, e)) % P
+    re1 = (_m31_mul(c, g) + P * P - _m31_mul(d, h)) % P  # (c + di)(g + hi)
+    im1 = (_m31_mul(c, h) + _m31_mul(d, g)) % P
+    re_r = (2 * re1 + P - im1) % P  # (2 + i)(re1 + im1 i)
+    im_r = (re1 + 2 * im1) % P
+    re2 = (_m31_mul(a, g) + P * P - _m31_mul(b, h) + _m31_mul(c, e) + P * P - _m31_mul(d, f)) % P
+    im2 = (_m31_mul(a, h) + _m31_mul(b, g) + _m31_mul(c, f) + _m31_mul(d, e)) % P
+    return np.stack([(re0 + re_r) % P, (im0 + im_r) % P, re2, im2], axis=1)
+
+
+def qm31_powers(alpha, n):
+    """alpha^0 .. alpha^(n-1) as an (n, 4) uint32 array (doubling blocks, one vector product each)."""
+    out = np.zeros((max(n, 1), 4), np.uint64)
+    out[0] = (1, 0, 0, 0)
+    step = np.array([alpha], np.uint64)  # alpha^(2^k)
+    have = 1
+    while have < n:
+        take = min(have, n - have)
+        out[have:have + take] = _qm_mul_vec(out[:take], np.repeat(step, take, axis=0))
+        have += take
+        step = _qm_mul_vec(step, step)
+    return out[:n].astype(np.uint32)
 
 
 # ------------------------------------------------------------------ layout (prover.hip)
@@ -191,17 +180,15 @@ def _root(layers):
     return [int(x) for x in layers[0].to_numpy().reshape(-1)]
 
 
-def _row(col, i):
-    """One value of a device column (a 4-byte device-to-host copy)."""
-    out = np.zeros(1, np.uint32)
-    _lib().copy_uint32_t_vec_from_device_to_host(col.ptr + 4 * i, out.ctypes.data, 1)
-    return int(out[0])
+def _rows(cols, rows):
+    """values[i, c] = cols[c][rows[i]], one gather call (Column::at for the decommitment)."""
+    return S.gather_rows(cols, rows)
 
 
-def _hash_row(layer, i):
-    out = np.zeros(8, np.uint32)
-    _lib().copy_blake_2s_hash_vec_from_device_to_host(layer.ptr + 32 * i, out.ctypes.data, 1)
-    return out
+def _paths(layers, leaf_log, leaves):
+    """Siblings of layers leaf_log - 1 .. 1 on the paths of the leaves, one call per tree."""
+    full = S.merkle_auth_paths([layers[k] for k in range(leaf_log + 1)], leaves)
+    return full[:, 1:, :].reshape(len(leaves), -1)
 
 
 def _lde(cols, log_n, tree):
@@ -234,7 +221,7 @@ def prove(n_cols, log_n, n_queries, pow_bits, last_log, seed):
     ch.mix(_root(layers_t))
     alpha = ch.draw_qm31()
 
-    powers = S.SecureFieldVec.from_numpy(np.array([S.qm_pow(alpha, k) for k in range(C_ - 2)], np.uint32))
+    powers = S.SecureFieldVec.from_numpy(qm31_powers(alpha, C_ - 2))
     dinv = S.BaseFieldVec.uninitialized(2 * n)
     S.check(_lib().stwo_hip_vanishing_inverse(dinv.ptr, log_n + 1, log_n))
     comp = S.SecureColumn.uninitialized(2 * n)
@@ -267,9 +254,7 @@ def prove(n_cols, log_n, n_queries, pow_bits, last_log, seed):
         lines.append(nxt)
     last = lines[-1].to_numpy()
     ch.mix(tree_root([int(last[k][i]) for i in range(1 << last_log) for k in range(4)]))
-    nonce = 0
-    while not pow_ok(ch.digest, nonce, pow_bits):
-        nonce += 1
+    nonce = S.grind(np.array(ch.digest, np.uint32), pow_bits)
     ch.mix([nonce & 0xFFFFFFFF, nonce >> 32])
     queries = ch.draw_queries(n_queries, log_n + 2)
 
@@ -283,23 +268,31 @@ def prove(n_cols, log_n, n_queries, pow_bits, last_log, seed):
     w[L["nonce"]:L["nonce"] + 2] = [nonce & 0xFFFFFFFF, nonce >> 32]
     w[L["queries"]:L["queries"] + n_queries] = queries
 
-    def path(layers, leaf_log, pos):
-        return np.concatenate([_hash_row(layers[k], (pos >> (leaf_log - k)) ^ 1) for k in range(leaf_log - 1, 0, -1)])
-
-    for q, Pq in enumerate(queries):
+    # decommitments: one gather per column set and one path gather per tree, for every query
+    qs = np.array(queries, np.uint32)
+    tr = _rows(lde, np.concatenate([qs >> 1, (qs >> 1) ^ 1]))
+    tp = _paths(layers_t, log_n + 1, qs >> 1)
+    cr = _rows(comp_lde, np.concatenate([qs, qs ^ 1]))
+    cp = _paths(layers_q, log_n + 2, qs)
+    lr, lp = [], []
+    for j in range(1, L["n_line"] + 1):
+        m = log_n + 2 - j
+        pj = qs >> j
+        lr.append(_rows(lines[j - 1].columns, np.concatenate([pj, pj ^ 1])))
+        lp.append(_paths(trees_l[j - 1], m, pj))
+    Qn = len(queries)
+    for q in range(Qn):
         b = L["per_query"] + q * L["query_words"]
-        R = Pq >> 1
-        w[b + L["q_trace_vals"]:b + L["q_trace_vals"] + C_] = [_row(c, R) for c in lde]
-        w[b + L["q_trace_vals"] + C_:b + L["q_trace_vals"] + 2 * C_] = [_row(c, R ^ 1) for c in lde]
-        w[b + L["q_trace_path"]:b + L["q_trace_path"] + 8 * log_n] = path(layers_t, log_n + 1, R)
-        w[b + L["q_comp_vals"]:b + L["q_comp_vals"] + 4] = [_row(c, Pq) for c in comp_lde]
-        w[b + L["q_comp_vals"] + 4:b + L["q_comp_vals"] + 8] = [_row(c, Pq ^ 1) for c in comp_lde]
-        w[b + L["q_comp_path"]:b + L["q_comp_path"] + 8 * (log_n + 1)] = path(layers_q, log_n + 2, Pq)
+        w[b + L["q_trace_vals"]:b + L["q_trace_vals"] + C_] = tr[q]
+        w[b + L["q_trace_vals"] + C_:b + L["q_trace_vals"] + 2 * C_] = tr[Qn + q]
+        w[b + L["q_trace_path"]:b + L["q_trace_path"] + 8 * log_n] = tp[q]
+        w[b + L["q_comp_vals"]:b + L["q_comp_vals"] + 4] = cr[q]
+        w[b + L["q_comp_vals"] + 4:b + L["q_comp_vals"] + 8] = cr[Qn + q]
+        w[b + L["q_comp_path"]:b + L["q_comp_path"] + 8 * (log_n + 1)] = cp[q]
         for j in range(1, L["n_line"] + 1):
             m = log_n + 2 - j
             o = b + L["q_line"][j - 1]
-            pj = Pq >> j
-            w[o:o + 4] = [_row(c, pj) for c in lines[j - 1].columns]
-            w[o + 4:o + 8] = [_row(c, pj ^ 1) for c in lines[j - 1].columns]
-            w[o + 8:o + 8 + 8 * (m - 1)] = path(trees_l[j - 1], m, pj)
+            w[o:o + 4] = lr[j - 1][q]
+            w[o + 4:o + 8] = lr[j - 1][Qn + q]
+            w[o + 8:o + 8 + 8 * (m - 1)] = lp[j - 1][q]
     return w
