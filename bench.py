@@ -721,8 +721,178 @@ def bench_serving(args):
                  extra=extra)
 
 
+class _Slab:
+    """One device allocation of n_cols columns of 2^log_n values back to back, with non-owning
+    column views (the reference benches' 2^16 small columns without one allocation each)."""
+
+    def __init__(self, S, n_cols, log_n):
+        self.buf = S.BaseFieldVec.uninitialized(n_cols << log_n)
+        self.cols = [S.BaseFieldVec.view(self.buf.ptr + (c << (log_n + 2)), 1 << log_n) for c in range(n_cols)]
+
+
+def bench_ref_interpolate(args):
+    """The reference's benches/interpolate_columns.rs shape: interpolate_columns of 2^16 columns of
+    2^10 values (in place, through the reference ABI with its host pointer array)."""
+    import stwo_hip as S
+
+    log_n, n_cols = 10, 1 << 16
+    slab = _Slab(S, n_cols, log_n)
+    S.check(S.lib().stwo_hip_fill_uniform_m31(slab.buf.ptr, n_cols << log_n, 0x5EED0008, 0))
+    coset = S.canonic_half_coset(log_n)
+    tree = S.precompute_twiddles(coset)
+    tw, itw = S.cached_twiddles(coset)
+    # the host pointer arrays are built once (a Rust caller passes its Vec of column pointers)
+    import numpy as np
+    ptrs = S._ptr_array([c.ptr for c in slab.cols])
+    logs = np.full(n_cols, log_n, np.uint32)
+    L = S.lib()
+
+    def step_ref():  # reference ABI: synchronous, uploads the pointer array itself
+        L.interpolate_columns(1 << (log_n - 1), ptrs, tree.itwiddles.ptr, tree.itwiddles.size, n_cols, 1 << log_n)
+
+    def step():
+        S.check(L.stwo_hip_interpolate_columns_async(ptrs, logs.ctypes.data_as(S.u32p), n_cols, itw, 1 << coset.log_size))
+    ms = _timed_steps(S, step_ref, args.steps, args.warmup)
+    ms_async = _timed_steps(S, step, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["fft_inv_tile"])
+    elems = n_cols << log_n
+    extra = {"kernels": ks, "ms_per_step_async_batched": round(ms_async, 4),
+             "note": "ms_per_step is the synchronous reference-ABI call (pointer-array upload + launch + sync)"}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        sample = 4096
+        host = O.fill_uniform_m31(sample << log_n, 1, 0).reshape(sample, -1)
+        otw, oitw = O.precompute_twiddles(coset.initial_index, coset.step_size, coset.log_size)
+        t0 = time.perf_counter()
+        O.lde_columns(list(host), 0, otw, oitw)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": (sample << log_n) / dt, "unit": "elems/s", "kind": "port", "cores": _cores(),
+                                 "sample": f"{sample} of the 2^16 columns; {dt:.3f} s"}
+    return _line("interpolate_columns M31 elems/s (2^16 columns x 2^10)", elems / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "reference bench interpolate_columns: 2^16 columns of 2^10, in place"},
+                 _roof(8 * elems, ks["fft_inv_tile"]["avg_ms"], "fft_inv_tile"), extra)
+
+
+def bench_ref_evaluate(args):
+    """The reference's benches/evaluate_columns.rs shape: evaluate_polynomials of 2^16 polynomials
+    of 2^10 coefficients at blowup 4 (domain 2^12).  value: the batched path from the coefficients
+    (the zero padding is implicit: no extend copy)."""
+    import stwo_hip as S
+
+    log_n, blow, n_cols = 10, 2, 1 << 16
+    m = log_n + blow
+    coef = _Slab(S, n_cols, log_n)
+    S.check(S.lib().stwo_hip_fill_uniform_m31(coef.buf.ptr, n_cols << log_n, 0x5EED0009, 0))
+    out = _Slab(S, n_cols, m)
+    big = S.canonic_half_coset(m)
+    tw, itw = S.cached_twiddles(big)
+    tl = 1 << big.log_size
+
+    # the host pointer arrays are built once (a Rust caller passes its Vec of column pointers)
+    import numpy as np
+    src, dst = S._ptr_array([c.ptr for c in coef.cols]), S._ptr_array([c.ptr for c in out.cols])
+    cl, el = np.full(n_cols, log_n, np.uint32), np.full(n_cols, m, np.uint32)
+    L = S.lib()
+
+    def step():
+        S.check(L.stwo_hip_evaluate_columns_async(src, cl.ctypes.data_as(S.u32p), dst, el.ctypes.data_as(S.u32p),
+                                                  n_cols, tw, tl))
+
+    ms = _timed_steps(S, step, args.steps, args.warmup)
+    ks = _kernel_avg(S, step, ["fft_fwd_tile"])
+    extra = {"kernels": ks}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        import numpy as np
+        sample = 512
+        host = O.fill_uniform_m31(sample << log_n, 2, 0).reshape(sample, -1)
+        otw, _ = O.precompute_twiddles(big.initial_index, big.step_size, big.log_size)
+        t0 = time.perf_counter()
+        for c in range(sample):
+            ext = np.zeros(1 << m, np.uint32)
+            ext[:1 << log_n] = host[c]
+            O.evaluate(ext, big.initial_index, otw)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": (sample << m) / dt, "unit": "elems/s", "kind": "port", "cores": 1,
+                                 "sample": f"{sample} of the 2^16 polynomials, one call per column; {dt:.3f} s"}
+    elems = n_cols << m
+    return _line("evaluate_polynomials M31 elems/s (2^16 x 2^10, blowup 4)", elems / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "reference bench evaluate_columns: 2^16 polynomials of 2^10 coefficients, blowup 4"},
+                 _roof(4 * (n_cols << log_n) + 4 * elems, ks["fft_fwd_tile"]["avg_ms"], "fft_fwd_tile"), extra)
+
+
+def bench_ref_bit_reverse(args):
+    """The reference's benches/bit_reverse.rs shape: bit_reverse_column of 2^28 M31 (value) and
+    2^28 QM31 (AoS, in the extra fields), in place."""
+    import stwo_hip as S
+
+    n = 1 << 28
+    res = {}
+    for secure in (False, True):
+        V = S.SecureFieldVec if secure else S.BaseFieldVec
+        col = V.uninitialized(n)
+        S.check(S.lib().stwo_hip_fill_uniform_m31(col.ptr, 4 * n if secure else n, 0x5EED0007, 0))
+        ms = _timed_steps(S, lambda: S.bit_reverse_column(col), args.steps, args.warmup)
+        ks = _kernel_avg(S, lambda: S.bit_reverse_column(col), ["bit_reverse"])
+        res["qm31" if secure else "m31"] = (ms, ks, (32 if secure else 8) * n)
+        del col
+    ms, ks, b = res["m31"]
+    extra = {"kernels": ks, "qm31": {"ms_per_step": round(res["qm31"][0], 4), "kernels": res["qm31"][1],
+                                     "roofline": _roof(res["qm31"][2], res["qm31"][1]["bit_reverse"]["avg_ms"],
+                                                       "bit_reverse")},
+             "note": "ms_per_step is the synchronous reference-ABI call"}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        host = O.fill_uniform_m31(1 << 26, 3, 0)
+        t0 = time.perf_counter()
+        O.bit_reverse(host)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": (1 << 26) / dt, "unit": "elems/s", "kind": "port", "cores": _cores(),
+                                 "sample": f"2^26 M31; {dt:.3f} s"}
+    return _line("bit_reverse M31 elems/s (2^28)", n / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "reference bench bit_reverse: 2^28 M31 in place (and 2^28 QM31)"},
+                 _roof(b, ks["bit_reverse"]["avg_ms"], "bit_reverse"), extra)
+
+
+def bench_ref_batch_inverse(args):
+    """The reference's benches/batch_inverse.rs shape: batch_inverse of 2^28 M31 (value) and 2^28
+    QM31 (in the extra fields)."""
+    import stwo_hip as S
+
+    n = 1 << 28
+    res = {}
+    for secure in (False, True):
+        V = S.SecureFieldVec if secure else S.BaseFieldVec
+        src, dst = V.uninitialized(n), V.uninitialized(n)
+        S.check(S.lib().stwo_hip_fill_uniform_m31(src.ptr, 4 * n if secure else n, 0x5EED0006, 0))
+        name = "batch_inverse_qm31" if secure else "batch_inverse_m31"
+        ms = _timed_steps(S, lambda: S.batch_inverse(src, dst), args.steps, args.warmup)
+        ks = _kernel_avg(S, lambda: S.batch_inverse(src, dst), [name])
+        res[name] = (ms, ks, (32 if secure else 8) * n)
+        del src, dst
+    ms, ks, b = res["batch_inverse_m31"]
+    q = res["batch_inverse_qm31"]
+    extra = {"kernels": ks, "qm31": {"ms_per_step": round(q[0], 4), "kernels": q[1],
+                                     "roofline": _roof(q[2], q[1]["batch_inverse_qm31"]["avg_ms"], "batch_inverse_qm31")},
+             "note": "ms_per_step is the synchronous reference-ABI call"}
+    if not args.no_cpu_baseline:
+        O = _oracle()
+        host = O.fill_uniform_m31(1 << 25, 4, 0)
+        host[host == 0] = 1
+        t0 = time.perf_counter()
+        O.batch_inverse_m31(host)
+        dt = time.perf_counter() - t0
+        extra["cpu_baseline"] = {"value": (1 << 25) / dt, "unit": "elems/s", "kind": "port", "cores": _cores(),
+                                 "sample": f"2^25 M31; {dt:.3f} s"}
+    return _line("batch_inverse M31 elems/s (2^28)", n / (ms * 1e-3), "elems/s", ms, args,
+                 {"workload": "reference bench batch_inverse: 2^28 M31 (and 2^28 QM31)"},
+                 _roof(b, ks["batch_inverse_m31"]["avg_ms"], "batch_inverse_m31"), extra)
+
+
 CONFIGS = {"lde-commit": bench_lde_commit, "small": bench_small, "batch-inverse": bench_batch_inverse,
-           "fri": bench_fri, "quotients": bench_quotients, "mixed-commit": bench_batched, "serving": bench_serving}
+           "fri": bench_fri, "quotients": bench_quotients, "mixed-commit": bench_batched, "serving": bench_serving,
+           "ref-interpolate": bench_ref_interpolate, "ref-evaluate": bench_ref_evaluate,
+           "ref-bit-reverse": bench_ref_bit_reverse, "ref-batch-inverse": bench_ref_batch_inverse}
 
 
 def main():

@@ -14,17 +14,67 @@
 namespace sh {
 namespace cols {
 
-constexpr int TA = 5;  // tile = 32 x 32 elements
-
+// tile = 2^TA x 2^TA elements: 64 x 64 for M31 (256-byte runs), 32 x 32 for QM31 (512-byte runs)
 template <class T>
-__global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* a, uint32_t log_n) {
-  __shared__ T ta[1 << (2 * TA)];
-  __shared__ T tb[1 << (2 * TA)];
+constexpr int tile_bits() {
+  return sizeof(T) == 4 ? 6 : 5;
+}
+
+constexpr uint32_t BR_NT = 256;
+// 4-byte elements: 64 x 64 tiles, every load of a thread in flight before its LDS stores
+template <class T>
+__global__ __launch_bounds__(BR_NT) void bitrev_tiled_kernel(T* a, uint32_t log_n) {
+  constexpr int TA = tile_bits<T>();
+  // LDS rows padded by one element, so the transposed reads hit distinct banks
+  constexpr uint32_t W = 1u << TA, RW = W + 1;
+  constexpr int PER = (int)(W * W / BR_NT);     // elements per thread and tile
+  __shared__ T ta[W * RW];
+  __shared__ T tb[W * RW];
   const uint32_t mid_bits = log_n - 2 * TA;
   const uint32_t m = blockIdx.x;
   const uint32_t mr = brev(m, mid_bits);
   if (mr < m) return;  // the pair is handled by block mr
-  const uint32_t W = 1u << TA;
+  const bool pair = mr != m;
+  auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
+    return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
+  };
+  // every load of the thread is issued before the first LDS store
+  T va[PER], vb[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + BR_NT * k, hi = e >> TA, lo = e & (W - 1);
+    va[k] = a[at(hi, m, lo)];
+    if (pair) vb[k] = a[at(hi, mr, lo)];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + BR_NT * k, hi = e >> TA, lo = e & (W - 1);
+    ta[hi * RW + lo] = va[k];
+    if (pair) tb[hi * RW + lo] = vb[k];
+  }
+  __syncthreads();
+  // element (hi, m, lo) goes to (brev lo, mr, brev hi)
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + BR_NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
+    const uint32_t src = brev(lo2, TA) * RW + brev(hi2, TA);
+    a[at(hi2, mr, lo2)] = ta[src];
+    if (pair) a[at(hi2, m, lo2)] = tb[src];
+  }
+}
+
+// 16-byte elements: 32 x 32 tiles (512-byte runs), unpadded LDS; measured faster with the
+// loads interleaved with the LDS stores than with the 4-byte kernel's load-all-first loop
+template <class T>
+__global__ __launch_bounds__(BR_NT) void bitrev_tiled16_kernel(T* a, uint32_t log_n) {
+  constexpr int TA = 5;
+  constexpr uint32_t W = 1u << TA;
+  __shared__ T ta[W * W];
+  __shared__ T tb[W * W];
+  const uint32_t mid_bits = log_n - 2 * TA;
+  const uint32_t m = blockIdx.x;
+  const uint32_t mr = brev(m, mid_bits);
+  if (mr < m) return;  // the pair is handled by block mr
   auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
     return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
   };
@@ -59,9 +109,13 @@ template <class T>
 void bit_reverse(T* a, uint64_t n) {
   require(is_pow2(n) && n <= (1ull << 31), "bit_reverse: size must be a power of two <= 2^31");
   uint32_t log_n = log2_exact(n);
+  constexpr int TA = tile_bits<T>();
   if (log_n >= 2 * TA) {
     timed("bit_reverse", [&] {
-      bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), 256, 0, ctx().cur>>>(a, log_n);
+      if constexpr (sizeof(T) == 4)
+        bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
+      else
+        bitrev_tiled16_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
     });
   } else if (n > 1) {
     timed("bit_reverse", [&] {
