@@ -770,7 +770,7 @@ def bench_ref_interpolate(args):
                                  "sample": f"{sample} of the 2^16 columns; {dt:.3f} s"}
     return _line("interpolate_columns M31 elems/s (2^16 columns x 2^10)", elems / (ms * 1e-3), "elems/s", ms, args,
                  {"workload": "reference bench interpolate_columns: 2^16 columns of 2^10, in place"},
-                 _roof(8 * elems, ks["fft_inv_tile"]["avg_ms"], "fft_inv_tile"), extra)
+                 _roof(8 * elems, ks["fft_inv_tile"]["avg_ms"], "fft_inv_tile", "interpolate_columns_2e16x2e10"), extra)
 
 
 def bench_ref_evaluate(args):
@@ -818,7 +818,8 @@ def bench_ref_evaluate(args):
     elems = n_cols << m
     return _line("evaluate_polynomials M31 elems/s (2^16 x 2^10, blowup 4)", elems / (ms * 1e-3), "elems/s", ms, args,
                  {"workload": "reference bench evaluate_columns: 2^16 polynomials of 2^10 coefficients, blowup 4"},
-                 _roof(4 * (n_cols << log_n) + 4 * elems, ks["fft_fwd_tile"]["avg_ms"], "fft_fwd_tile"), extra)
+                 _roof(4 * (n_cols << log_n) + 4 * elems, ks["fft_fwd_tile"]["avg_ms"], "fft_fwd_tile",
+                       "evaluate_polynomials_2e16x2e10_blowup4"), extra)
 
 
 def bench_ref_bit_reverse(args):
@@ -839,7 +840,7 @@ def bench_ref_bit_reverse(args):
     ms, ks, b = res["m31"]
     extra = {"kernels": ks, "qm31": {"ms_per_step": round(res["qm31"][0], 4), "kernels": res["qm31"][1],
                                      "roofline": _roof(res["qm31"][2], res["qm31"][1]["bit_reverse"]["avg_ms"],
-                                                       "bit_reverse")},
+                                                       "bit_reverse_qm31", "bit_reverse_2e28")},
              "note": "ms_per_step is the synchronous reference-ABI call"}
     if not args.no_cpu_baseline:
         O = _oracle()
@@ -851,7 +852,7 @@ def bench_ref_bit_reverse(args):
                                  "sample": f"2^26 M31; {dt:.3f} s"}
     return _line("bit_reverse M31 elems/s (2^28)", n / (ms * 1e-3), "elems/s", ms, args,
                  {"workload": "reference bench bit_reverse: 2^28 M31 in place (and 2^28 QM31)"},
-                 _roof(b, ks["bit_reverse"]["avg_ms"], "bit_reverse"), extra)
+                 _roof(b, ks["bit_reverse"]["avg_ms"], "bit_reverse", "bit_reverse_2e28"), extra)
 
 
 def bench_ref_batch_inverse(args):
@@ -873,7 +874,8 @@ def bench_ref_batch_inverse(args):
     ms, ks, b = res["batch_inverse_m31"]
     q = res["batch_inverse_qm31"]
     extra = {"kernels": ks, "qm31": {"ms_per_step": round(q[0], 4), "kernels": q[1],
-                                     "roofline": _roof(q[2], q[1]["batch_inverse_qm31"]["avg_ms"], "batch_inverse_qm31")},
+                                     "roofline": _roof(q[2], q[1]["batch_inverse_qm31"]["avg_ms"], "batch_inverse_qm31",
+                                                       "batch_inverse_2e28")},
              "note": "ms_per_step is the synchronous reference-ABI call"}
     if not args.no_cpu_baseline:
         O = _oracle()
@@ -886,7 +888,7 @@ def bench_ref_batch_inverse(args):
                                  "sample": f"2^25 M31; {dt:.3f} s"}
     return _line("batch_inverse M31 elems/s (2^28)", n / (ms * 1e-3), "elems/s", ms, args,
                  {"workload": "reference bench batch_inverse: 2^28 M31 (and 2^28 QM31)"},
-                 _roof(b, ks["batch_inverse_m31"]["avg_ms"], "batch_inverse_m31"), extra)
+                 _roof(b, ks["batch_inverse_m31"]["avg_ms"], "batch_inverse_m31", "batch_inverse_2e28"), extra)
 
 
 CONFIGS = {"lde-commit": bench_lde_commit, "small": bench_small, "batch-inverse": bench_batch_inverse,
