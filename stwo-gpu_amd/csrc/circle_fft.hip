@@ -58,7 +58,15 @@ struct PassArgs {
   uint32_t n_groups;     // column groups (grid x = n_groups * tiles_per_group)
   uint32_t band;         // consecutive blocks walk `band` columns at one tile, then the next tile
   uint32_t lob_count_log;  // log2 of the number of low blocks per column (a - t)
+  // block order (CONTIG / STRIDED, set by set_block_order): tiles per column = 2^tiles_log;
+  // the last band holds band_last columns; *_magic: ceil(2^32 / d) for the divisions by
+  // band and band_last (0 for d = 1), exact for every numerator the order forms
+  uint32_t tiles_log, band_last, band_magic, band_last_magic;
 };
+
+// ceil(2^32 / d) for 2 <= d, or 0 for d = 1: divu(n, d, m) = n / d while n (m d - 2^32) < 2^32
+inline uint32_t div_magic(uint32_t d) { return d == 1 ? 0u : (uint32_t)(((1ull << 32) + d - 1) / d); }
+__device__ __forceinline__ uint32_t divu(uint32_t n, uint32_t m) { return m ? __umulhi(n, m) : n; }
 
 // accelerator complex dies (XCDs) of gfx950, each with its own L2
 constexpr uint32_t N_XCD = 8;
@@ -110,15 +118,17 @@ struct Pass {
       // (block i to XCD i % 8), each with its own L2, so each XCD takes a contiguous range of
       // this order: the band members of one tile then run on one XCD and read its twiddles
       // from that XCD's L2 once.
+      // (The divisions by band sizes multiply by host-computed reciprocals, on the scalar
+      // unit; a division by a runtime value costs ~5 VALU instructions per wave.)
       const uint32_t G = gridDim.x, q = G / N_XCD, rem = G % N_XCD;
       const uint32_t x = blockIdx.x % N_XCD;
       const uint32_t v = x * q + min(x, rem) + blockIdx.x / N_XCD;
-      const uint32_t tiles = G / A.n_groups;
-      const uint32_t per_band = A.band * tiles;
-      const uint32_t bi = v / per_band, r = v - bi * per_band;
-      const uint32_t w = min(A.band, A.n_groups - bi * A.band);
-      col_group = bi * A.band + r % w;
-      tile = r / w;
+      const uint32_t bi = divu(v >> A.tiles_log, A.band_magic);
+      const uint32_t r = v - ((bi * A.band) << A.tiles_log);
+      const bool last = (bi + 1) * A.band >= A.n_groups;
+      const uint32_t w = last ? A.band_last : A.band;
+      tile = divu(r, last ? A.band_last_magic : A.band_magic);
+      col_group = bi * A.band + (r - tile * w);
     }
     if constexpr (KIND == MULTI) {
       block_base = 0;
@@ -224,7 +234,11 @@ struct Pass {
   template <int LO, int W>
   __device__ __forceinline__ void load(uint32_t (&v)[E]) const {
     constexpr int R = 1 << W;
-    const uint32_t smask = (1u << A.src_log) - 1;
+    // only masked forward passes read from fewer coefficients than the domain (src_log <
+    // log_n, the first pass of an evaluation from 2^act_log coefficients); every other
+    // CONTIG / STRIDED pass reads whole columns, and without the mask its addresses are one
+    // add per load.  MULTI tiles always mask: it also drops the column bits of p.
+    const uint32_t smask = (MASKED || KIND == MULTI) ? (1u << A.src_log) - 1 : ~0u;
 #pragma unroll
     for (int gi = 0; gi < E / R; gi++) {
       const uint32_t pb = spread<T + LO, W>(threadIdx.x + NT * gi);
@@ -235,7 +249,9 @@ struct Pass {
         for (int r = 0; r < R; r++) v[gi * R + r] = c ? c[(pb + (r << LO)) & smask] : 0u;
       } else if constexpr (LO == 0 && T == 0 && W == 3) {
         const uint32_t i0 = index_of(pb);
-        if (A.src_log >= 3) {
+        // unmasked passes read whole domains (src_log = log_n >= 3): no per-element fallback,
+        // so the compiler keeps the two 16-byte loads instead of merging both paths
+        if (!MASKED || A.src_log >= 3) {
           const SH_GLOBAL uint4* q = reinterpret_cast<const SH_GLOBAL uint4*>(src);
           const uint32_t qi = (i0 & smask) >> 2;
           uint4 x0 = at(q, qi), x1 = at(q, qi + 1);
@@ -532,6 +548,26 @@ struct PassPlan {
   uint32_t a, k, t, hb, multi, B;
 };
 
+// Block order of a CONTIG / STRIDED launch over A.n_cols columns of 2^tiles_log tiles each.
+static void set_block_order(PassArgs& A, uint32_t tiles_log) {
+  A.n_groups = A.n_cols;
+  A.band = std::min(A.n_cols, FFT_BAND);
+  A.tiles_log = tiles_log;
+  const uint32_t n_bands = ceil_div(A.n_cols, A.band);
+  A.band_last = A.n_cols - (n_bands - 1) * A.band;
+  A.band_magic = div_magic(A.band);
+  A.band_last_magic = div_magic(A.band_last);
+  // exactness of divu for the numerators formed: column indexes (< n_cols) and positions
+  // within a band (< band 2^tiles_log)
+  const uint64_t max_r = ((uint64_t)A.band << tiles_log);
+  auto exact = [](uint64_t n_max, uint32_t d, uint32_t m) {
+    return m == 0 || n_max * ((uint64_t)m * d - (1ull << 32)) < (1ull << 32);
+  };
+  require(tiles_log < 31 && max_r < (1ull << 31) && exact(A.n_cols, A.band, A.band_magic) &&
+              exact(max_r, A.band, A.band_magic) && exact(max_r, A.band_last, A.band_last_magic),
+          "fft: block order out of range");
+}
+
 // kc_pref: layers in the contiguous pass (12 or 13; 0 = the fewest passes, then 12).
 static std::vector<PassPlan> plan(uint32_t log_n, uint32_t kc_pref = 0) {
   std::vector<PassPlan> ps;
@@ -604,10 +640,9 @@ static void launch_pass(const PassPlan& p, const PassArgs& base) {
     A.lob_count_log = 0;
     blocks = A.n_groups;
   } else {
-    A.n_groups = A.n_cols;
-    A.band = std::min(A.n_cols, FFT_BAND);
     A.lob_count_log = p.a - p.t;
     uint32_t hi_log = A.log_n - p.a - p.k - p.hb;
+    set_block_order(A, A.lob_count_log + hi_log);
     blocks = A.n_cols << (A.lob_count_log + hi_log);
   }
   timed(name, [&] {
@@ -724,9 +759,8 @@ void lde_same_size(const uint32_t* const* d_src, uint32_t* const* d_coeffs, uint
   Ai.hb = 0;
   Ai.multi = 0;
   Ai.scale = 31 - n;
-  Ai.n_groups = n_cols;
-  Ai.band = std::min(n_cols, FFT_BAND);
   Ai.lob_count_log = 12 - t;
+  set_block_order(Ai, 12 - t);
   PassArgs Af = Ai;
   Af.src = (const uint32_t* const*)d_lde;
   Af.dst = d_lde;
