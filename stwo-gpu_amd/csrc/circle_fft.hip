@@ -261,11 +261,23 @@ struct Pass {
 #pragma unroll
           for (int r = 0; r < 8; r++) v[gi * 8 + r] = at(src, (i0 + r) & smask);
         }
+      } else if constexpr (!MASKED) {
+        // value r of the group sits r << (LO + a) words after value 0 (pb has zeros in the
+        // group's bits): one vector offset for the group and a scalar one per r
+        const uint32_t i0 = index_of(pb);
+        const BufCol b(src);
+#pragma unroll
+        for (int r = 0; r < R; r++) v[gi * R + r] = b.load(i0, r_offset<LO>(r));
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) v[gi * R + r] = at(src, index_of(pb | (r << (T + LO))) & smask);
       }
     }
+  }
+  // word offset of value r of a radix group at stage LO from the group's first value (uniform)
+  template <int LO>
+  __device__ __forceinline__ uint32_t r_offset(int r) const {
+    return (uint32_t)r << (LO + (KIND == STRIDED ? A.a : 0u));
   }
 
   template <int LO, int W>
@@ -293,8 +305,10 @@ struct Pass {
         at(q, qi) = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
         at(q, qi + 1) = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
       } else {
+        const uint32_t i0 = index_of(pb);
+        const BufCol b(dst);
 #pragma unroll
-        for (int r = 0; r < R; r++) at(dst, index_of(pb | (r << (T + LO)))) = v[gi * R + r];
+        for (int r = 0; r < R; r++) b.store(v[gi * R + r], i0, r_offset<LO>(r));
       }
     }
   }

@@ -31,4 +31,30 @@ __device__ __forceinline__ const SH_GLOBAL T& at(const SH_GLOBAL T* base, uint32
   return *(const SH_GLOBAL T*)((const SH_GLOBAL char*)base + i * (uint32_t)sizeof(T));
 }
 
+// One column as a raw buffer (a scalar resource descriptor): loads and stores take a
+// vector byte offset and a scalar one, so accesses of a thread that differ by uniform
+// strides share one vector offset and cost no vector address arithmetic (global_* accesses
+// take only a scalar base plus a vector offset, and per-access uniform offsets become
+// 64-bit vector adds).  The base must be uniform; byte offsets stay below 2^32.
+struct BufCol {
+  __amdgpu_buffer_rsrc_t rs;
+  // num_records 0xffffffff: no range clamp below 4 GiB; dword data format (word 3).  The
+  // base goes through readfirstlane so the descriptor stays in SGPRs even where the
+  // compiler cannot see that the pointer is uniform (else every access becomes a loop).
+  __device__ __forceinline__ explicit BufCol(const void* p) : rs(make(p)) {}
+  static __device__ __forceinline__ __amdgpu_buffer_rsrc_t make(const void* p) {
+    const uint64_t a = (uint64_t)p;
+    const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)u, (short)0, -1, 0x00020000);
+  }
+  // word i + s (i per lane, s uniform)
+  __device__ __forceinline__ uint32_t load(uint32_t i, uint32_t s) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, i * 4u, s * 4u, 0);
+  }
+  __device__ __forceinline__ void store(uint32_t v, uint32_t i, uint32_t s) const {
+    __builtin_amdgcn_raw_buffer_store_b32(v, rs, i * 4u, s * 4u, 0);
+  }
+};
+
 }  // namespace sh
