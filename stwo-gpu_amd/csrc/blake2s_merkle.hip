@@ -74,31 +74,37 @@ __device__ __forceinline__ void compress(uint32_t (&h)[8], const uint32_t (&m)[1
 }
 
 // Absorb the columns of one layer at `row` (16-column chunks, zero-padded remainder).
-// jo: the job offset of the column pointers (see jptr)
+// jo: the job offset of the column pointers (see jptr).  The column table is read through
+// the scalar cache and each column becomes a scalar buffer descriptor with row * 4 as the
+// lane's offset (global loads of pointer + jo + row * 4 compile to a 64-bit vector add per
+// load, and generic-pointer reads to flat loads).
+__device__ __forceinline__ uint32_t column_word(const uint32_t* const* cols, uint32_t c, uint32_t row, uint64_t jo) {
+  return BufCol(jptr(ld_uniform(cols, c), jo)).load(row, 0);
+}
 __device__ __forceinline__ void absorb_columns_simple(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
                                                       uint32_t row, uint64_t jo) {
   uint32_t c = 0;
   for (; c + 16 <= n_cols; c += 16) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = jptr(cols[c + j], jo)[row];
+    for (int j = 0; j < 16; j++) m[j] = column_word(cols, c + j, row, jo);
     compress(h, m);
   }
   if (c < n_cols) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? jptr(cols[c + j], jo)[row] : 0u;
+    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? column_word(cols, c + j, row, jo) : 0u;
     compress(h, m);
   }
 }
 
-__device__ __forceinline__ void store_hash(Hash* out, uint32_t i, const uint32_t (&h)[8]) {
-  uint4* o = reinterpret_cast<uint4*>(out + i);
+__device__ __forceinline__ void store_hash(SH_GLOBAL Hash* out, uint32_t i, const uint32_t (&h)[8]) {
+  SH_GLOBAL uint4* o = reinterpret_cast<SH_GLOBAL uint4*>(out + i);
   o[0] = make_uint4(h[0], h[1], h[2], h[3]);
   o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
-__device__ __forceinline__ void load_children(const Hash* prev, uint32_t i, uint32_t (&m)[16]) {
-  const uint4* p = reinterpret_cast<const uint4*>(prev + 2 * (size_t)i);
+__device__ __forceinline__ void load_children(const SH_GLOBAL Hash* prev, uint32_t i, uint32_t (&m)[16]) {
+  const SH_GLOBAL uint4* p = reinterpret_cast<const SH_GLOBAL uint4*>(prev + 2 * (size_t)i);
   uint4 a = p[0], b = p[1], c = p[2], d = p[3];
   m[0] = a.x; m[1] = a.y; m[2] = a.z; m[3] = a.w;
   m[4] = b.x; m[5] = b.y; m[6] = b.z; m[7] = b.w;
@@ -116,11 +122,11 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
   uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (prev) {
     uint32_t m[16];
-    load_children(jptr(prev, jo), i, m);
+    load_children(as_global(jptr(prev, jo)), i, m);
     compress(h, m);
   }
   absorb_columns_simple(h, cols, n_cols, i, jo);
-  store_hash(jptr(out, jo), i, h);
+  store_hash(as_global(jptr(out, jo)), i, h);
 }
 
 
@@ -213,8 +219,8 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
     const uint32_t layer = top - d;
     const uint32_t cnt = nodes0 >> d;
     const uint32_t base = base0 >> d;
-    const LayerCols lc = inj[layer];
-    Hash* const out = jptr(outs[layer], jo);
+    const LayerCols lc = ld_uniform(inj, layer);
+    SH_GLOBAL Hash* const out = as_global(jptr(outs[layer], jo));
     if (d > 0 && cnt <= NT / 4) {
       // narrow level: one quad per node, children read from s in place
       const uint32_t i = threadIdx.x >> 2, q = threadIdx.x & 3;
@@ -227,7 +233,7 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const uint32_t col = c + 4 * q + k;
-            mb[4 * q + k] = col < lc.n_cols ? jptr(lc.cols[col], jo)[base + i] : 0u;
+            mb[4 * q + k] = col < lc.n_cols ? at(as_global(jptr(at(as_global(lc.cols), col), jo)), base + i) : 0u;
           }
           __builtin_amdgcn_wave_barrier();
           quad_compress(h_lo, h_hi, mb, q);
@@ -252,7 +258,7 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
         if (d > 0 || prev) {  // leaves (no prev) have no children to compress
           uint32_t m[16];
           if (d == 0) {
-            load_children(prev, base + i, m);
+            load_children(as_global(prev), base + i, m);
           } else {
 #pragma unroll
             for (int w = 0; w < 8; w++) {

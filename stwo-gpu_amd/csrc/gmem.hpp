@@ -10,8 +10,10 @@
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SH_GLOBAL __attribute__((address_space(1)))
+#define SH_CONST __attribute__((address_space(4)))
 #else
 #define SH_GLOBAL
+#define SH_CONST
 #endif
 
 namespace sh {
@@ -19,6 +21,24 @@ namespace sh {
 template <class T>
 __device__ __forceinline__ SH_GLOBAL T* as_global(T* p) {
   return (SH_GLOBAL T*)p;
+}
+
+// A uniform pointer passed through readfirstlane: the compiler then sees an SGPR value and
+// can no longer fold uniform terms of it (a job offset, say) into a lane's 64-bit offset,
+// which would turn at() into a 64-bit vector add per access.
+template <class T>
+__device__ __forceinline__ T* uniform(T* p) {
+  const uint64_t a = (uint64_t)p;
+  return (T*)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+              (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a));
+}
+
+// p[i] read through the scalar cache into SGPRs: p and i uniform, and nothing this kernel
+// writes (tables uploaded before the launch).  A generic pointer read in a kernel that also
+// stores is otherwise a vector (flat) load, even at a uniform address.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p, uint32_t i) {
+  return ((const SH_CONST T*)p)[i];
 }
 
 // element i of base; i * sizeof(T) must be below 2^32
@@ -43,10 +63,7 @@ struct BufCol {
   // compiler cannot see that the pointer is uniform (else every access becomes a loop).
   __device__ __forceinline__ explicit BufCol(const void* p) : rs(make(p)) {}
   static __device__ __forceinline__ __amdgpu_buffer_rsrc_t make(const void* p) {
-    const uint64_t a = (uint64_t)p;
-    const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
-                       (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
-    return __builtin_amdgcn_make_buffer_rsrc((void*)u, (short)0, -1, 0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(uniform(p)), (short)0, -1, 0x00020000);
   }
   // word i + s (i per lane, s uniform)
   __device__ __forceinline__ uint32_t load(uint32_t i, uint32_t s) const {
