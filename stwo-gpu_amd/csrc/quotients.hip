@@ -124,9 +124,10 @@ __global__ __launch_bounds__(256) void accumulate_kernel(DomainTables T, uint32_
 // once and multiplied into the sums of all NB batches, with alpha = 0 for a batch that
 // does not sample the column.  Per column the host provides NB alphas (one uniform
 // 16 * NB-byte record), so the loop has no data-dependent branches; loads run one group
-// of QG columns ahead of the arithmetic.  Sums are folded after every fourth column:
-// a product of two values below P is at most (P - 1)^2 = 2^62 - 2^33 + 4, and a folded
-// sum is below 2^34, so four products on top of it stay below 2^64.
+// of QG columns ahead of the arithmetic.  A batch's sums are folded after every fourth
+// product taken into them: a product of two values below P is at most (P - 1)^2 =
+// 2^62 - 2^33 + 1, and a folded sum is below 2^34, so four products on top of it stay
+// below 2^64.
 template <int NB>
 struct ColSlot {
   const uint32_t* col;  // the column (device pointer)
@@ -171,6 +172,12 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, ui
   // slots follow, so the next group's loads are issued unconditionally: a conditional
   // prefetch makes the compiler wait for every load in flight before the arithmetic
   uint32_t v[QG], vn[QG];
+  // products taken into each batch's sums since their last fold (uniform): a batch's sums
+  // are folded after its fourth product, not after every fourth column, so batches that
+  // sample few of the columns fold rarely
+  uint32_t since[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) since[b] = 0;
 #pragma unroll
   for (int i = 0; i < QG; i++) v[i] = at(as_global(slots[i].col), row);
   for (uint32_t h = 0; h < n_slots; h += QG) {
@@ -186,10 +193,8 @@ __global__ __launch_bounds__(256) void accumulate_cols_kernel(DomainTables T, ui
         s[b][1] += (uint64_t)S.alpha[b].x.b * v[i];
         s[b][2] += (uint64_t)S.alpha[b].y.a * v[i];
         s[b][3] += (uint64_t)S.alpha[b].y.b * v[i];
-      }
-      if (i % 4 == 3) {
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
+        if (++since[b] == 4) {
+          since[b] = 0;
           s[b][0] = fold64(s[b][0]);
           s[b][1] = fold64(s[b][1]);
           s[b][2] = fold64(s[b][2]);
