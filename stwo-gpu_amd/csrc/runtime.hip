@@ -188,17 +188,34 @@ static Staging& take_staging(size_t bytes) {
   }
   Staging st{};
   st.cap = std::max<size_t>(bytes, 64 << 10);
-  hip_check(hipHostMalloc(&st.host, st.cap, hipHostMallocDefault), "hipHostMalloc");
+  // coherent (fine-grained): small uploads are read by a kernel straight from this memory,
+  // so the device must not keep lines of an earlier upload cached
+  hip_check(hipHostMalloc(&st.host, st.cap, hipHostMallocCoherent), "hipHostMalloc");
   hip_check(hipEventCreateWithFlags(&st.done, hipEventDisableTiming), "hipEventCreate");
   c.staging.push_back(st);
   return c.staging.back();
 }
 
+// Small uploads: a kernel on the current stream reads the staging buffer over the bus.  A
+// DMA copy would be ordered with the stream's kernels through a cross-engine wait whose
+// latency (tens of microseconds) every following kernel sees.
+__global__ void upload_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n4) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) dst[i] = src[i];
+}
+constexpr size_t UPLOAD_KERNEL_MAX = 64 << 10;
+
 void upload_to(void* dev, const void* host, size_t bytes) {
   if (!bytes) return;
   Staging& st = take_staging(bytes);
   memcpy(st.host, host, bytes);
-  hip_check(hipMemcpyAsync(dev, st.host, bytes, hipMemcpyHostToDevice, g_ctx.cur), "hipMemcpyAsync H2D");
+  if (bytes <= UPLOAD_KERNEL_MAX && bytes % 4 == 0 && ((uintptr_t)dev | (uintptr_t)st.host) % 4 == 0) {
+    const uint32_t n4 = (uint32_t)(bytes / 4);
+    upload_kernel<<<(n4 + 255) / 256, 256, 0, g_ctx.cur>>>((uint32_t*)dev, (const uint32_t*)st.host, n4);
+    hip_check(hipGetLastError(), "upload_kernel");
+  } else {
+    hip_check(hipMemcpyAsync(dev, st.host, bytes, hipMemcpyHostToDevice, g_ctx.cur), "hipMemcpyAsync H2D");
+  }
   hip_check(hipEventRecord(st.done, g_ctx.cur), "hipEventRecord");
   st.busy = true;
 }

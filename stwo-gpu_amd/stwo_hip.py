@@ -650,11 +650,18 @@ class QuotientSamples:
         self.vals = (QM31 * max(1, len(self.idx)))(*[qm31(v) for _, cv in sample_batches for _, v in cv])
 
 
+def column_pointers(columns):
+    """Host array of the columns' device pointers, for calls repeated over the same columns."""
+    return _ptr_array([c.ptr for c in columns])
+
+
 def accumulate_quotients_async(domain_half_coset: Coset, columns, random_coeff, samples: QuotientSamples,
-                               out: SecureColumn):
+                               out: SecureColumn, col_ptrs=None):
+    """col_ptrs: column_pointers(columns), built once by a caller that repeats the call."""
     n = 2 * domain_half_coset.size()
     _c(lib().stwo_hip_accumulate_quotients_async(
-        domain_half_coset.initial_index, domain_half_coset.step_size, n, _ptr_array([c.ptr for c in columns]),
+        domain_half_coset.initial_index, domain_half_coset.step_size, n,
+        col_ptrs if col_ptrs is not None else column_pointers(columns),
         len(columns), qm31(random_coeff), samples.n, C.cast(samples.points, vp), samples.sizes.ctypes.data_as(u32p),
         samples.idx.ctypes.data_as(u32p), samples.vals, out.ptrs()))
 
