@@ -825,6 +825,15 @@ void lde_mixed(const uint32_t* const* src, uint32_t* const* coeffs, uint32_t* co
 // holds the inputs when they are not transformed in place.
 void interpolate_mixed(uint32_t* const* cols, const uint32_t* logs, uint32_t n_cols, const uint32_t* itw,
                        uint64_t tree_len, const uint32_t* const* src = nullptr) {
+  // one size (the common case, e.g. 2^16 columns of one trace): the caller's pointer table
+  // is uploaded as it is, once when the columns are transformed in place
+  if (n_cols && std::all_of(logs, logs + n_cols, [&](uint32_t l) { return l == logs[0]; })) {
+    DevBuf table = upload(cols, (size_t)n_cols * sizeof(void*));
+    DevBuf stable = src ? upload(src, (size_t)n_cols * sizeof(void*)) : DevBuf();
+    interpolate_same_size(table.as<uint32_t*>(), n_cols, logs[0], itw, tree_len,
+                          src ? stable.as<const uint32_t*>() : table.as<const uint32_t*>());
+    return;
+  }
   std::map<uint32_t, std::pair<std::vector<uint32_t*>, std::vector<const uint32_t*>>> by;
   for (uint32_t i = 0; i < n_cols; i++) {
     by[logs[i]].first.push_back(cols[i]);
@@ -840,6 +849,17 @@ void interpolate_mixed(uint32_t* const* cols, const uint32_t* logs, uint32_t n_c
 }
 void evaluate_mixed(const uint32_t* const* src, const uint32_t* src_logs, uint32_t* const* dst,
                     const uint32_t* dst_logs, uint32_t n_cols, const uint32_t* tw, uint64_t tree_len) {
+  // one (source, destination) size pair: the caller's tables as they are, one upload when
+  // the columns are evaluated in place
+  if (n_cols && std::all_of(src_logs, src_logs + n_cols, [&](uint32_t l) { return l == src_logs[0]; }) &&
+      std::all_of(dst_logs, dst_logs + n_cols, [&](uint32_t l) { return l == dst_logs[0]; })) {
+    const bool in_place = std::equal(src, src + n_cols, (const uint32_t* const*)dst);
+    DevBuf td = upload(dst, (size_t)n_cols * sizeof(void*));
+    DevBuf ts = in_place ? DevBuf() : upload(src, (size_t)n_cols * sizeof(void*));
+    evaluate_same_size(td.as<uint32_t*>(), in_place ? td.as<const uint32_t*>() : ts.as<const uint32_t*>(), n_cols,
+                       src_logs[0], dst_logs[0], tw, tree_len);
+    return;
+  }
   std::map<std::pair<uint32_t, uint32_t>, std::pair<std::vector<const uint32_t*>, std::vector<uint32_t*>>> by;
   for (uint32_t i = 0; i < n_cols; i++) {
     auto& e = by[{src_logs[i], dst_logs[i]}];
