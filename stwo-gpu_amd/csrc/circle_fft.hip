@@ -64,7 +64,7 @@ struct PassArgs {
   uint32_t tiles_log, band_last, band_magic, band_last_magic;
 };
 
-// ceil(2^32 / d) for 2 <= d, or 0 for d = 1: divu(n, d, m) = n / d while n (m d - 2^32) < 2^32
+// m = ceil(2^32 / d) for 2 <= d, or 0 for d = 1: divu(n, m) = n / d while n (m d - 2^32) < 2^32
 inline uint32_t div_magic(uint32_t d) { return d == 1 ? 0u : (uint32_t)(((1ull << 32) + d - 1) / d); }
 __device__ __forceinline__ uint32_t divu(uint32_t n, uint32_t m) { return m ? __umulhi(n, m) : n; }
 

@@ -23,9 +23,9 @@ __device__ __forceinline__ SH_GLOBAL T* as_global(T* p) {
   return (SH_GLOBAL T*)p;
 }
 
-// A uniform pointer passed through readfirstlane: the compiler then sees an SGPR value and
-// can no longer fold uniform terms of it (a job offset, say) into a lane's 64-bit offset,
-// which would turn at() into a 64-bit vector add per access.
+// A pointer that is the same in every lane, passed through readfirstlane: where the compiler
+// cannot prove it uniform (a value carried through a loop, say) the result still lives in
+// SGPRs; where it can, the readfirstlane folds away.
 template <class T>
 __device__ __forceinline__ T* uniform(T* p) {
   const uint64_t a = (uint64_t)p;
