@@ -221,12 +221,13 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
     const uint32_t base = base0 >> d;
     const LayerCols lc = ld_uniform(inj, layer);
     SH_GLOBAL Hash* const out = as_global(jptr(outs[layer], jo));
-    if (d > 0 && cnt <= NT / 4) {
-      // narrow level: one quad per node, children read from s in place
+    if ((d > 0 || !prev) && cnt <= NT / 4) {
+      // narrow level: one quad per node, children read from s in place (a leaf level of at
+      // most NT / 4 nodes too: it has no children, only its columns)
       const uint32_t i = threadIdx.x >> 2, q = threadIdx.x & 3;
       uint32_t h_lo = 0, h_hi = 0;
       if (i < cnt) {
-        quad_compress(h_lo, h_hi, s[2 * i].s, q);
+        if (d > 0) quad_compress(h_lo, h_hi, s[2 * i].s, q);
         uint32_t* mb = qmsg[i];
         for (uint32_t c = 0; c < lc.n_cols; c += 16) {
           __builtin_amdgcn_wave_barrier();
