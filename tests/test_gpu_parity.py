@@ -227,6 +227,24 @@ def test_lde_async_matches_oracle(log_n, blow):
         assert (ldes[c].to_numpy() == olde[c]).all()
 
 
+@pytest.mark.parametrize("logs", [[10, 10, 10, 10], [5, 9, 9, 12, 3]])
+def test_interpolate_columns_out_of_place(logs):
+    """stwo_hip_interpolate_columns_oop_async: coefficients into separate buffers, sources untouched
+    (one size takes the single-table upload, several sizes the per-size grouping)."""
+    root = S.canonic_half_coset(max(logs))
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << l, 500 + i) for i, l in enumerate(logs)]
+    src = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    dst = [S.BaseFieldVec.uninitialized(1 << l) for l in logs]
+    S.interpolate_columns_oop_async(src, dst, logs, itw, 1 << root.log_size)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    for i, l in enumerate(logs):
+        oc, _ = O.lde_columns([evals[i]], 0, otw, oitw)
+        assert (dst[i].to_numpy() == oc[0]).all(), f"column {i} (log {l})"
+        assert (src[i].to_numpy() == evals[i]).all(), f"source {i} changed"
+
+
 def test_lde_mixed_sizes_one_call():
     logs = [4, 13, 9, 13, 16, 4, 11]
     root = S.canonic_half_coset(17)
