@@ -94,6 +94,8 @@ static Soa4 soa(uint32_t* const* p) { return Soa4{{p[0], p[1], p[2], p[3]}}; }
 constexpr uint32_t CHAIN_MAX_S = 13;  // 2^13 inputs per block: 5 x 2^12 words of LDS (80 KiB)
 constexpr int CHAIN_NT = 1024;  // two blocks (32 waves) per CU, 64 VGPRs at most
 constexpr int CHAIN_ROUND = 2;  // first-fold outputs per thread whose loads are in flight together
+constexpr uint32_t CHAIN_SMALL_S = 10;  // tiles of at most 2^10 inputs: 256 threads
+constexpr int CHAIN_SMALL_NT = 256;
 
 struct ChainArgs {
   const uint32_t* itw;  // the whole inverse tree
@@ -121,7 +123,11 @@ __device__ __forceinline__ qm31 fold_pair(qm31 a, qm31 b, uint32_t t, qm31 alpha
   return qadd(qadd(a, b), qmul(alpha, qscale(qsub(a, b), t)));
 }
 
-__global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) void fold_chain_kernel(ChainArgs A) {
+// NT threads for tiles of at most 2^SMAX inputs (the wide launches: 1024 and 2^13; a chain's
+// last layers, at most 2^10 values in one block: 256, which keeps the barriers of its
+// short folds cheap)
+template <int NT, int SMAX>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void fold_chain_kernel(ChainArgs A) {
   extern __shared__ uint32_t lds[];
   const uint32_t h0 = 1u << (A.S - 1);  // outputs of the block's first fold
   uint32_t* const pl[4] = {lds, lds + h0, lds + 2 * h0, lds + 3 * h0};
@@ -129,12 +135,12 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
   // block, its twiddles at twl[h0 - 2 cnt_j, h0 - cnt_j); the loads are issued first and stored
   // to LDS after the first fold, so their latency overlaps that fold's loads
   uint32_t* const twl = lds + 4 * h0;
-  constexpr int PRE = (1 << (CHAIN_MAX_S - 1)) / CHAIN_NT;
+  constexpr int PRE = (1 << (SMAX - 1)) / NT;
   uint32_t v[PRE];
   {
 #pragma unroll
     for (int q = 0; q < PRE; q++) {
-      const uint32_t u = threadIdx.x + q * CHAIN_NT;
+      const uint32_t u = threadIdx.x + q * NT;
       if (u + 1 < h0) {
         const uint32_t r = h0 - u;                     // in (h0 >> j, h0 >> (j - 1)]
         const uint32_t j = A.S - (32 - __clz(r - 1));  // so that ceil(log2 r) = S - j
@@ -152,13 +158,13 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
     const uint32_t g0 = blockIdx.x * h0;
     // CHAIN_ROUND outputs per thread per round: every load of a round is issued before the
     // first store (the stores could alias the loads, so they would otherwise serialise)
-    for (uint32_t k0 = 0; k0 < h0; k0 += CHAIN_ROUND * CHAIN_NT) {
+    for (uint32_t k0 = 0; k0 < h0; k0 += CHAIN_ROUND * NT) {
       uint2 w[CHAIN_ROUND][4];
       uint32_t t[CHAIN_ROUND];
       qm31 d[CHAIN_ROUND];
 #pragma unroll
       for (int q = 0; q < CHAIN_ROUND; q++) {
-        const uint32_t k = k0 + threadIdx.x + q * CHAIN_NT, g = g0 + k;
+        const uint32_t k = k0 + threadIdx.x + q * NT, g = g0 + k;
         if (k < h0) {  // pair g of the input: values 2g, 2g + 1 (one 8-byte load per coordinate)
 #pragma unroll
           for (int c = 0; c < 4; c++)
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
       }
 #pragma unroll
       for (int q = 0; q < CHAIN_ROUND; q++) {
-        const uint32_t k = k0 + threadIdx.x + q * CHAIN_NT, g = g0 + k;
+        const uint32_t k = k0 + threadIdx.x + q * NT, g = g0 + k;
         if (k < h0) {
           const qm31 a = {{w[q][0].x, w[q][1].x}, {w[q][2].x, w[q][3].x}};
           const qm31 b = {{w[q][0].y, w[q][1].y}, {w[q][2].y, w[q][3].y}};
@@ -186,12 +192,12 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
   }
 #pragma unroll
   for (int q = 0; q < PRE; q++) {
-    const uint32_t u = threadIdx.x + q * CHAIN_NT;
+    const uint32_t u = threadIdx.x + q * NT;
     if (u + 1 < h0 && A.S - (32 - __clz(h0 - u - 1)) < A.n_fold) twl[u] = v[q];
   }
   __syncthreads();
   // the later folds: the block's previous outputs from LDS, twiddles from the read-ahead
-  constexpr int PER = (1 << (CHAIN_MAX_S - 2)) / CHAIN_NT;  // outputs per thread of the second fold
+  constexpr int PER = (1 << (SMAX - 2)) / NT;  // outputs per thread of the second fold
   for (uint32_t j = 1; j < A.n_fold; j++) {
     const uint32_t cnt = h0 >> j;
     const uint32_t* tw = twl + (h0 - 2 * cnt);
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
     qm31 r[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const uint32_t k = threadIdx.x + q * CHAIN_NT;
+      const uint32_t k = threadIdx.x + q * NT;
       if (k < cnt) {
         const qm31 a = {{pl[0][2 * k], pl[1][2 * k]}, {pl[2][2 * k], pl[3][2 * k]}};
         const qm31 b = {{pl[0][2 * k + 1], pl[1][2 * k + 1]}, {pl[2][2 * k + 1], pl[3][2 * k + 1]}};
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(CHAIN_NT) __attribute__((amdgpu_waves_per_eu(8))) v
     __syncthreads();  // every read of the previous layer is done
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const uint32_t k = threadIdx.x + q * CHAIN_NT;
+      const uint32_t k = threadIdx.x + q * NT;
       if (k < cnt) {
         st(A.dst[j], g0 + k, r[q]);
         pl[0][k] = r[q].x.a;
@@ -264,7 +270,12 @@ void fri_fold_chain_async(const uint32_t* itw, uint64_t tree_len, uint32_t log_c
       A.dst[f] = fri::soa(layers + 4 * (size_t)(j + f));
     }
     timed(log_cur > S ? "fri_fold_chain" : "fri_fold_chain_tail", [&] {
-      fri::fold_chain_kernel<<<1u << (log_cur - S), fri::CHAIN_NT, 20u << (S - 1), ctx().cur>>>(A);
+      if (S <= fri::CHAIN_SMALL_S)
+        fri::fold_chain_kernel<fri::CHAIN_SMALL_NT, fri::CHAIN_SMALL_S>
+            <<<1u << (log_cur - S), fri::CHAIN_SMALL_NT, 20u << (S - 1), ctx().cur>>>(A);
+      else
+        fri::fold_chain_kernel<fri::CHAIN_NT, fri::CHAIN_MAX_S>
+            <<<1u << (log_cur - S), fri::CHAIN_NT, 20u << (S - 1), ctx().cur>>>(A);
     });
     j += nf;
     log_cur -= nf;
