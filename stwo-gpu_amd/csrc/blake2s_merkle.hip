@@ -211,16 +211,42 @@ __global__ __launch_bounds__(256) void quad_layer_kernel(uint32_t size, uint32_t
 // lane per node; levels of at most NT/4 nodes one lane quad per node (the four G
 // functions of each half-round side by side).  outs[k] and inj[k] are indexed by layer k;
 // s holds nodes0 hashes.
+// Leaf r of a LeafFold (lf.src[0] non-null): computed from the folded layer, written to
+// the leaf layer's four columns (cols, job offset jo) by lane `writer` (all lanes when it
+// is 4), and returned.
+__device__ __forceinline__ qm31 leaf_fold_value(const LeafFold& lf, const uint32_t* const* cols, uint32_t r,
+                                                uint64_t jo, uint32_t writer, uint32_t lane) {
+  const qm31 al = *jptr(lf.alpha, jo);
+  qm31 a, b;
+  uint32_t* ac[4] = {&a.x.a, &a.x.b, &a.y.a, &a.y.b};
+  uint32_t* bc[4] = {&b.x.a, &b.x.b, &b.y.a, &b.y.b};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const uint2 w = at(reinterpret_cast<const SH_GLOBAL uint2*>(as_global(jptr(lf.src[c], jo))), r);
+    *ac[c] = w.x;
+    *bc[c] = w.y;
+  }
+  const qm31 f = qadd(qadd(a, b), qmul(al, qscale(qsub(a, b), at(as_global(lf.tw), r))));
+  const uint32_t fw[4] = {f.x.a, f.x.b, f.y.a, f.y.b};
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    if (writer == 4 || (uint32_t)c == lane)
+      at(as_global(jptr(const_cast<uint32_t*>(ld_uniform(cols, c)), jo)), r) = fw[c];  // the plan's leaf columns
+  return f;
+}
+
 template <int NT>
 __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uint32_t depth, uint32_t nodes0,
                                              uint32_t base0, Hash* const* outs, const LayerCols* inj, Hash* s,
-                                             uint32_t (*qmsg)[16], uint64_t jo) {
+                                             uint32_t (*qmsg)[16], uint64_t jo, const LeafFold& lf) {
   for (uint32_t d = 0; d < depth; d++) {
     const uint32_t layer = top - d;
     const uint32_t cnt = nodes0 >> d;
     const uint32_t base = base0 >> d;
     const LayerCols lc = ld_uniform(inj, layer);
     SH_GLOBAL Hash* const out = as_global(jptr(outs[layer], jo));
+    // the leaf level of a LeafFold launch: four columns, computed here (prev is null)
+    const bool fold = d == 0 && lf.src[0] != nullptr;
     if ((d > 0 || !prev) && cnt <= NT / 4) {
       // narrow level: one quad per node, children read from s in place (a leaf level of at
       // most NT / 4 nodes too: it has no children, only its columns)
@@ -229,7 +255,17 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
       if (i < cnt) {
         if (d > 0) quad_compress(h_lo, h_hi, s[2 * i].s, q);
         uint32_t* mb = qmsg[i];
-        for (uint32_t c = 0; c < lc.n_cols; c += 16) {
+        if (fold) {
+          // each lane of the quad folds the leaf, stores its own coordinate and stages it
+          const qm31 f = leaf_fold_value(lf, lc.cols, base + i, jo, q, q);
+          const uint32_t fw[4] = {f.x.a, f.x.b, f.y.a, f.y.b};
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int k = 0; k < 4; k++) mb[4 * q + k] = q == 0 ? fw[k] : 0u;
+          __builtin_amdgcn_wave_barrier();
+          quad_compress(h_lo, h_hi, mb, q);
+        }
+        for (uint32_t c = fold ? lc.n_cols : 0; c < lc.n_cols; c += 16) {
           __builtin_amdgcn_wave_barrier();
 #pragma unroll
           for (int k = 0; k < 4; k++) {
@@ -269,7 +305,13 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
           }
           compress(h, m);
         }
-        absorb_columns_simple(h, lc.cols, lc.n_cols, base + i, jo);
+        if (fold) {
+          const qm31 f = leaf_fold_value(lf, lc.cols, base + i, jo, 4, 0);
+          uint32_t m[16] = {f.x.a, f.x.b, f.y.a, f.y.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+          compress(h, m);
+        } else {
+          absorb_columns_simple(h, lc.cols, lc.n_cols, base + i, jo);
+        }
         store_hash(out, base + i, h);
       }
       __syncthreads();  // this round's reads of s are done
@@ -304,7 +346,7 @@ constexpr int SUB_NT = 512;  // threads per subtree block (measured: 512 ahead o
 template <int NT>
 __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t top, uint32_t depth, uint32_t w_log,
                                                      Hash* const* outs, const LayerCols* inj, RootDraw rd,
-                                                     uint32_t* arrive, uint64_t js) {
+                                                     uint32_t* arrive, uint64_t js, LeafFold lf) {
   extern __shared__ Hash s[];  // max(2^w_log, block count / 2) hashes
   __shared__ uint32_t qmsg[NT / 4][16];  // quad mode: staging of injected column chunks
   __shared__ uint32_t is_last;
@@ -318,7 +360,7 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
     rd.dst = jptr(rd.dst, jo);
   }
   const uint32_t bd = arrive ? w_log + 1 : depth;  // levels built by every block
-  build_levels<NT>(prev, top, bd, 1u << w_log, blockIdx.x << w_log, outs, inj, s, qmsg, jo);
+  build_levels<NT>(prev, top, bd, 1u << w_log, blockIdx.x << w_log, outs, inj, s, qmsg, jo, lf);
   if (arrive) {
     __threadfence();  // this block's nodes are visible device-wide before it counts in
     __syncthreads();
@@ -328,7 +370,8 @@ __global__ __launch_bounds__(NT) void subtree_kernel(const Hash* prev, uint32_t 
     __threadfence();       // every other block's nodes are visible to the loads below
     if (threadIdx.x == 0) *arrive = 0u;
     const uint32_t t_top = top - bd;  // the block roots are layer t_top + 1
-    build_levels<NT>(jptr(outs[t_top + 1], jo), t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg, jo);
+    build_levels<NT>(jptr(outs[t_top + 1], jo), t_top, depth - bd, gridDim.x >> 1, 0, outs, inj, s, qmsg, jo,
+                     LeafFold{});
   }
   // the launch that wrote the root (layer 0, held in s[0] after the last barrier) runs
   // the channel step on one lane quad: Channel::mix8 and Channel::draw_qm31 / draw_point
@@ -425,13 +468,19 @@ void MerklePlan::build(uint32_t n_cols, const uint32_t* const* cols, const uint3
   }
 }
 
-bool MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready, const RootDraw* root_draw) const {
+bool MerklePlan::enqueue(const std::vector<hipEvent_t>* layer_ready, const RootDraw* root_draw,
+                         const LeafFold* leaf_fold) const {
   size_t next = 0;
-  return enqueue_from(&next, 0, layer_ready, root_draw);
+  return enqueue_from(&next, 0, layer_ready, root_draw, leaf_fold);
+}
+
+bool MerklePlan::leaf_fold_ok() const {
+  return !steps.empty() && steps[0].kind == Step::SUBTREE && steps[0].top == max_log && lcs[max_log].n_cols == 4;
 }
 
 bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready,
-                              const RootDraw* root_draw) const {
+                              const RootDraw* root_draw, const LeafFold* leaf_fold) const {
+  require(!leaf_fold || (leaf_fold_ok() && *next == 0), "merkle: leaf fold on a plan that cannot take it");
   hipStream_t st = ctx().cur;
   bool drawn = false;
   for (; *next < steps.size() && lowest_layer(steps[*next]) >= min_layer; ++*next) {
@@ -476,10 +525,11 @@ bool MerklePlan::enqueue_from(size_t* next, uint32_t min_layer, const std::vecto
         rd = *root_draw;
         drawn = true;
       }
+      const LeafFold lf = (leaf_fold && s.top == max_log) ? *leaf_fold : LeafFold{};
       timed("merkle_subtree", [&] {
         subtree_kernel<SUB_NT><<<dim3(blocks, 1, n_jobs), SUB_NT, lds, st>>>(
             s.top == max_log ? nullptr : outs[s.top + 1], s.top, s.depth, w_log, douts, dlcs, rd,
-            tail ? darrive + (size_t)s.top * n_jobs : nullptr, jstride);
+            tail ? darrive + (size_t)s.top * n_jobs : nullptr, jstride, lf);
       });
     }
   }

@@ -26,6 +26,18 @@ struct RootDraw {
   uint32_t point;
 };
 
+// A leaf layer that is one FRI line fold of another layer (a proof's FRI layer trees): the
+// launch that hashes the leaves computes them, leaf r = (a + b) + alpha (a - b) tw[r] with
+// a, b = src[2r], src[2r + 1] (QM31 in four coordinate columns), writes them to the leaf
+// layer's four columns and hashes them, so the fold needs no launch of its own.  src and
+// alpha are job 0's (they move by the job offset), alpha is read from device memory (the
+// previous commitment's channel step drew it).
+struct LeafFold {
+  const uint32_t* src[4];
+  const uint32_t* tw;  // the folded layer's line twiddles, tw[r] for leaf r
+  const qm31* alpha;
+};
+
 // Columns injected into a layer: device array of column pointers.
 struct LayerCols {
   const uint32_t* const* cols;
@@ -70,13 +82,18 @@ struct MerklePlan {
   // columns are still being produced.
   // root_draw (optional): run by the subtree launch that writes the root; returns whether
   // it ran (false when the root is written by a layer launch: the caller mixes it itself).
-  bool enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr, const RootDraw* root_draw = nullptr) const;
+  // leaf_fold (optional, leaf_fold_ok()): the leaves are computed by that fold
+  bool enqueue(const std::vector<hipEvent_t>* layer_ready = nullptr, const RootDraw* root_draw = nullptr,
+               const LeafFold* leaf_fold = nullptr) const;
+  // whether enqueue can compute the leaves by a LeafFold: the leaf layer holds exactly four
+  // columns (one QM31 column) and one subtree launch hashes it
+  bool leaf_fold_ok() const;
   // Enqueue steps from *next on while every layer they read is >= min_layer (the caller
   // has enqueued the producers of those layers); advances *next.  Interleaving these
   // calls with the producers' launches hands the hashing to the device as early as the
   // host submits its inputs.
   bool enqueue_from(size_t* next, uint32_t min_layer, const std::vector<hipEvent_t>* layer_ready,
-                    const RootDraw* root_draw = nullptr) const;
+                    const RootDraw* root_draw = nullptr, const LeafFold* leaf_fold = nullptr) const;
   // lowest layer a step reads injected columns of
   static uint32_t lowest_layer(const Step& s) { return s.kind == Step::LAYER ? s.top : s.top + 1 - s.depth; }
 

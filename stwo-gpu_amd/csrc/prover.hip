@@ -618,15 +618,27 @@ void Job::enqueue(const uint64_t* seeds) {
   auto l1 = coords(line[0], ln + 1);
   fold_circle_into_line_async(itw, tree_len - (1ull << (ln + 1)), 4 * n, f2c, qzero(), l1.data(), &jb->fri_alpha[0],
                               false, K, js);
+  // line[k] = fold(line[k - 1], alpha_k) for k >= 2 is computed by the launch that hashes
+  // line[k]'s leaves when its tree is one subtree launch (LeafFold), saving a fold launch
+  // per layer; line[1] also accumulates the circle fold of f1, and line[n_line] has no tree
+  auto fused = [&](uint32_t k) { return k >= 2 && k < L.n_line && tree_l[k]->leaf_fold_ok(); };
   for (uint32_t j = 1; j <= L.n_line; j++) {
     const uint32_t m = ln + 2 - j;
     const merkle::RootDraw rd{&jb->ch, &jb->fri_alpha[j], 0};
-    if (!tree_l[j - 1]->enqueue(nullptr, &rd))
+    merkle::LeafFold lf{};
+    if (fused(j - 1)) {
+      auto prev_src = coords(line[j - 2], m + 1);
+      for (int k = 0; k < 4; k++) lf.src[k] = prev_src[k];
+      lf.tw = itw + (tree_len - (1ull << (m + 1)));
+      lf.alpha = &jb->fri_alpha[j - 1];
+    }
+    if (!tree_l[j - 1]->enqueue(nullptr, &rd, fused(j - 1) ? &lf : nullptr))
       timed("channel_fri_root",
             [&] { k_mix_root_draw<<<one, 1, 0, stream>>>(jb, layers_l[j - 1][0], &jb->fri_alpha[j], js); });
     auto src = coords(line[j - 1], m), dst = coords(line[j], m - 1);
-    fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j], K,
-                    js);
+    if (!fused(j))
+      fold_line_async(itw, tree_len - (1ull << m), 1ull << m, src.data(), qzero(), dst.data(), &jb->fri_alpha[j],
+                      K, js);
     if (j == 1)
       fold_circle_into_line_async(itw, tree_len - (1ull << ln), 2 * n, f1c, qzero(), dst.data(), &jb->fri_alpha[1],
                                   true, K, js);
