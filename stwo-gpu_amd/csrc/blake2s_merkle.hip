@@ -78,22 +78,25 @@ __device__ __forceinline__ void compress(uint32_t (&h)[8], const uint32_t (&m)[1
 // the scalar cache and each column becomes a scalar buffer descriptor with row * 4 as the
 // lane's offset (global loads of pointer + jo + row * 4 compile to a 64-bit vector add per
 // load, and generic-pointer reads to flat loads).
-__device__ __forceinline__ uint32_t column_word(const uint32_t* const* cols, uint32_t c, uint32_t row, uint64_t jo) {
-  return BufCol(jptr(ld_uniform(cols, c), jo)).load(row, 0);
+// Row row_base + row_off: row_base uniform (the descriptor starts there, so the lane offsets
+// stay small), row_off per lane.
+__device__ __forceinline__ uint32_t column_word(const uint32_t* const* cols, uint32_t c, uint32_t row_base,
+                                                uint32_t row_off, uint64_t jo) {
+  return BufCol(jptr(ld_uniform(cols, c), jo) + row_base).load(row_off, 0);
 }
 __device__ __forceinline__ void absorb_columns_simple(uint32_t (&h)[8], const uint32_t* const* cols, uint32_t n_cols,
-                                                      uint32_t row, uint64_t jo) {
+                                                      uint32_t row_base, uint32_t row_off, uint64_t jo) {
   uint32_t c = 0;
   for (; c + 16 <= n_cols; c += 16) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = column_word(cols, c + j, row, jo);
+    for (int j = 0; j < 16; j++) m[j] = column_word(cols, c + j, row_base, row_off, jo);
     compress(h, m);
   }
   if (c < n_cols) {
     uint32_t m[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? column_word(cols, c + j, row, jo) : 0u;
+    for (int j = 0; j < 16; j++) m[j] = (c + j < n_cols) ? column_word(cols, c + j, row_base, row_off, jo) : 0u;
     compress(h, m);
   }
 }
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(256) void layer_kernel(uint32_t size, uint32_t n_co
     load_children(as_global(jptr(prev, jo)), i, m);
     compress(h, m);
   }
-  absorb_columns_simple(h, cols, n_cols, i, jo);
+  absorb_columns_simple(h, cols, n_cols, blockIdx.x * blockDim.x, threadIdx.x, jo);
   store_hash(as_global(jptr(out, jo)), i, h);
 }
 
@@ -310,7 +313,7 @@ __device__ __forceinline__ void build_levels(const Hash* prev, uint32_t top, uin
           uint32_t m[16] = {f.x.a, f.x.b, f.y.a, f.y.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
           compress(h, m);
         } else {
-          absorb_columns_simple(h, lc.cols, lc.n_cols, base + i, jo);
+          absorb_columns_simple(h, lc.cols, lc.n_cols, base + r0, threadIdx.x, jo);
         }
         store_hash(out, base + i, h);
       }

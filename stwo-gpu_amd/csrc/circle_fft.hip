@@ -264,8 +264,8 @@ struct Pass {
       } else if constexpr (!MASKED) {
         // value r of the group sits r << (LO + a) words after value 0 (pb has zeros in the
         // group's bits): one vector offset for the group and a scalar one per r
-        const uint32_t i0 = index_of(pb);
-        const BufCol b(src);
+        const BufCol b(src + block_base);
+        const uint32_t i0 = index_of(pb) - block_base;
 #pragma unroll
         for (int r = 0; r < R; r++) v[gi * R + r] = b.load(i0, r_offset<LO>(r));
       } else {
@@ -305,8 +305,8 @@ struct Pass {
         at(q, qi) = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
         at(q, qi + 1) = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
       } else {
-        const uint32_t i0 = index_of(pb);
-        const BufCol b(dst);
+        const BufCol b(dst + block_base);
+        const uint32_t i0 = index_of(pb) - block_base;
 #pragma unroll
         for (int r = 0; r < R; r++) b.store(v[gi * R + r], i0, r_offset<LO>(r));
       }

@@ -55,7 +55,10 @@ __device__ __forceinline__ const SH_GLOBAL T& at(const SH_GLOBAL T* base, uint32
 // vector byte offset and a scalar one, so accesses of a thread that differ by uniform
 // strides share one vector offset and cost no vector address arithmetic (global_* accesses
 // take only a scalar base plus a vector offset, and per-access uniform offsets become
-// 64-bit vector adds).  The base must be uniform; byte offsets stay below 2^32.
+// 64-bit vector adds).  The base must be uniform.  The descriptor's range is 2^32 - 1
+// bytes and an access must end inside it: word offsets from the base stay below 2^30 - 1,
+// so callers put the base at a block's first word rather than at a column's (the last
+// word of a 2^30-word column is out of range from the column's start).
 struct BufCol {
   __amdgpu_buffer_rsrc_t rs;
   // num_records 0xffffffff: no range clamp below 4 GiB; dword data format (word 3).  The

@@ -302,3 +302,39 @@ def test_evaluate_polynomials_reference_bench_shape():
     S.check(S.lib().stwo_hip_memcpy(back.buf.ptr, got.ctypes.data, got.nbytes))
     S.interpolate_columns(back.cols, tree)
     assert (back.to_numpy(m) == ext).all()
+
+
+# ------------------------------------------------------------------ the largest columns: log 29 and 30
+@pytest.mark.parametrize("log_n", [29, 30])
+def test_fft_round_trip_largest_columns(log_n):
+    """Columns of 2^29 and 2^30 values (byte offsets up to 2^32 - 4, every pass shape of the
+    largest plan): evaluate a sparse coefficient vector into a buffer of unrelated values, then
+    interpolate out of place into another; the result must be exactly the sparse vector, so a
+    tile that either pass skips or misplaces shows up as nonzero words or wrong values."""
+    n = 1 << log_n
+    rng = np.random.default_rng(0x5EED0010 + log_n)
+    idx = np.unique(np.concatenate([rng.integers(0, n, 61), [0, 1, n // 2, n - 1]])).astype(np.int64)
+    vals = rng.integers(1, P, idx.size).astype(np.uint32)
+    coef = S.BaseFieldVec.zeros(n)
+    for i, v in zip(idx, vals):
+        w = np.array([v], np.uint32)
+        S.check(S.lib().stwo_hip_memcpy(coef.ptr + 4 * int(i), w.ctypes.data, 4))
+    evals = S.BaseFieldVec.uninitialized(n)
+    S.fill_uniform(evals, 0x5EED0011, 0)
+    back = S.BaseFieldVec.uninitialized(n)
+    S.fill_uniform(back, 0x5EED0012, 0)
+    root = S.canonic_half_coset(log_n)
+    tw, itw = S.cached_twiddles(root)
+    S.evaluate_columns_async([coef], [log_n], [evals], [log_n], tw, 1 << root.log_size)
+    S.interpolate_columns_oop_async([evals], [back], [log_n], itw, 1 << root.log_size)
+    S.synchronize()
+    del evals, coef
+    got = back.to_numpy()
+    del back
+    nz = np.flatnonzero(got)
+    assert np.array_equal(nz, idx), f"nonzero positions differ: {nz[:8]} vs {idx[:8]}"
+    assert np.array_equal(got[idx], vals)
+    del got
+    gc.collect()
+    S.check(S.lib().stwo_hip_release_twiddles())  # the log-30 trees hold 4 GiB
+
