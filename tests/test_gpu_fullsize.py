@@ -338,3 +338,45 @@ def test_fft_round_trip_largest_columns(log_n):
     gc.collect()
     S.check(S.lib().stwo_hip_release_twiddles())  # the log-30 trees hold 4 GiB
 
+
+# ------------------------------------------------------------------ Merkle layers of 2^30 and 2^29 nodes
+def _hash_rows(h, lo, count):
+    """count hashes of a device hash vector from row lo (a partial device-to-host copy)."""
+    out = np.empty(8 * count, np.uint32)
+    S.check(S.lib().stwo_hip_memcpy(out.ctypes.data, h.ptr + 32 * lo, 32 * count))
+    return out.reshape(-1, 8)
+
+
+def test_merkle_layers_2e30_and_2e29_rows():
+    """The largest Merkle layers: commit_on_first_layer over two columns of 2^30 rows and
+    commit_on_layer_with_previous over one column of 2^29 rows; the first and last rows of
+    each layer (where a column's byte offsets reach 2^32 - 4) against BLAKE2s from the oracle."""
+    L = 30
+    cols = [S.BaseFieldVec.uninitialized(1 << L) for _ in range(2)]
+    for c, col in enumerate(cols):
+        S.fill_uniform(col, 0x5EED0013, c << 31)
+    top = S.commit_on_layer(L, None, cols)
+    col29 = S.BaseFieldVec.uninitialized(1 << (L - 1))
+    S.fill_uniform(col29, 0x5EED0014, 0)
+    nxt = S.commit_on_layer(L - 1, top, [col29])
+    S.synchronize()
+
+    def leaf(vals):
+        return O.blake2s_compress(np.zeros(8, np.uint32), np.array(list(vals) + [0] * (16 - len(vals)), np.uint32))
+
+    for lo, cnt in [(0, 4), ((1 << L) - 64, 64)]:
+        vals = [O.fill_uniform_m31(cnt, 0x5EED0013, (c << 31) + lo) for c in range(2)]
+        got = _hash_rows(top, lo, cnt)
+        for r in range(cnt):
+            assert (got[r] == leaf([vals[0][r], vals[1][r]])).all(), f"leaf row {lo + r}"
+    for lo, cnt in [(0, 4), ((1 << (L - 1)) - 32, 32)]:
+        children = _hash_rows(top, 2 * lo, 2 * cnt)
+        vals = O.fill_uniform_m31(cnt, 0x5EED0014, lo)
+        got = _hash_rows(nxt, lo, cnt)
+        for r in range(cnt):
+            st = O.blake2s_compress(np.zeros(8, np.uint32), np.concatenate([children[2 * r], children[2 * r + 1]]))
+            st = O.blake2s_compress(st, np.array([vals[r]] + [0] * 15, np.uint32))
+            assert (got[r] == st).all(), f"node row {lo + r}"
+    del top, nxt, cols, col29
+    gc.collect()
+
