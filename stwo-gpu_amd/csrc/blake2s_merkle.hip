@@ -229,7 +229,7 @@ __device__ __forceinline__ qm31 leaf_fold_value(const LeafFold& lf, const uint32
     *ac[c] = w.x;
     *bc[c] = w.y;
   }
-  const qm31 f = qadd(qadd(a, b), qmul(al, qscale(qsub(a, b), at(as_global(lf.tw), r))));
+  const qm31 f = fold_mat(a, b, at(as_global(lf.tw), r), qmat_of(al));
   const uint32_t fw[4] = {f.x.a, f.x.b, f.y.a, f.y.b};
 #pragma unroll
   for (int c = 0; c < 4; c++)

@@ -114,6 +114,67 @@ __host__ __device__ __forceinline__ qm31 qpow(qm31 p, uint64_t e) {
   }
   return r;
 }
+// ---- products by a fixed QM31 factor (FRI folds: one alpha per layer) ------------------
+// alpha z = alpha_x z_x + (2 + i) alpha_y z_y + (alpha_x z_y + alpha_y z_x) u is linear in
+// z = (z_x.a, z_x.b, z_y.a, z_y.b): a 4 x 4 matrix over M31 with entries in [0, P), so each
+// output coordinate is four 64-bit multiply-adds and one reduction (a product by qmul costs
+// nine reductions and the Karatsuba additions).  With alpha = (a0 + a1 i) + (a2 + a3 i) u and
+// (2 + i)(a2 + a3 i) = b0 + b1 i the rows are
+//   [a0, -a1, b0, -b1], [a1, a0, b1, b0], [a2, -a3, a0, -a1], [a3, a2, a1, a0]
+// and only their nine distinct entries are kept (uniform values live in scalar registers).
+struct qmat {
+  uint32_t a0, a1, na1, a2, a3, na3, b0, b1, nb1;
+};
+__host__ __device__ __forceinline__ qmat qmat_of(qm31 a) {
+  const cm31 b = cmul_r(a.y);
+  return {a.x.a, a.x.b, neg(a.x.b), a.y.a, a.y.b, neg(a.y.b), b.a, b.b, neg(b.b)};
+}
+// y < 2^64 reduced mod P: 2^32 = 2 (mod P), so y = 2 hi + lo (mod P), below 3 2^32; one more
+// fold of that gives at most 2^31 + 4 <= 2P
+__host__ __device__ __forceinline__ uint32_t reduce64(uint64_t y) {
+  const uint64_t z = (uint64_t)(uint32_t)(y >> 32) * 2u + (uint32_t)y;
+  const uint32_t lo = (uint32_t)z, hi = (uint32_t)(z >> 32);
+  return canon((lo & P) + (lo >> 31) + 2 * hi);
+}
+// a - b for canonical a, b: a - b wraps above 2^31 exactly when a < b, and then a - b + P
+// is the smaller of the two
+__host__ __device__ __forceinline__ uint32_t sub3(uint32_t a, uint32_t b) {
+  const uint32_t x = a - b, y = x + P;
+  return x < y ? x : y;
+}
+__host__ __device__ __forceinline__ uint32_t dot4(uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3,
+                                                  const uint32_t e[4], uint32_t c) {
+  // c < 2^32 plus four products of values below P: at most 4 (P - 1)^2 + 2^32 < 2^64
+  return reduce64((uint64_t)c + (uint64_t)m0 * e[0] + (uint64_t)m1 * e[1] + (uint64_t)m2 * e[2] +
+                  (uint64_t)m3 * e[3]);
+}
+// alpha e + c coordinatewise (c[k] < 2^32), reduced
+__host__ __device__ __forceinline__ qm31 qmat_apply(const qmat& M, const uint32_t e[4], const uint32_t c[4]) {
+  return {{dot4(M.a0, M.na1, M.b0, M.nb1, e, c[0]), dot4(M.a1, M.a0, M.b1, M.b0, e, c[1])},
+          {dot4(M.a2, M.na3, M.a0, M.na1, e, c[2]), dot4(M.a3, M.a2, M.a1, M.a0, e, c[3])}};
+}
+// FRI fold of a pair: (a + b) + alpha t (a - b), alpha given by its matrix (t canonical)
+__host__ __device__ __forceinline__ qm31 fold_mat(qm31 a, qm31 b, uint32_t t, const qmat& M) {
+  const uint32_t t2 = t + t;
+  const uint32_t e[4] = {mul_dbl(sub3(a.x.a, b.x.a), t2), mul_dbl(sub3(a.x.b, b.x.b), t2),
+                         mul_dbl(sub3(a.y.a, b.y.a), t2), mul_dbl(sub3(a.y.b, b.y.b), t2)};
+  const uint32_t c[4] = {a.x.a + b.x.a, a.x.b + b.x.b, a.y.a + b.y.a, a.y.b + b.y.b};
+  return qmat_apply(M, e, c);
+}
+// the circle fold accumulated into acc: acc alpha^2 + (a + b) + alpha t (a - b), M2 the
+// matrix of alpha^2 (the acc alpha^2 + a + b part reduced first, so each sum stays below 2^64)
+__host__ __device__ __forceinline__ qm31 fold_mat_acc(qm31 a, qm31 b, uint32_t t, const qmat& M, qm31 acc,
+                                                     const qmat& M2) {
+  const uint32_t ac[4] = {acc.x.a, acc.x.b, acc.y.a, acc.y.b};
+  const uint32_t c[4] = {a.x.a + b.x.a, a.x.b + b.x.b, a.y.a + b.y.a, a.y.b + b.y.b};
+  const qm31 s = qmat_apply(M2, ac, c);
+  const uint32_t t2 = t + t;
+  const uint32_t e[4] = {mul_dbl(sub3(a.x.a, b.x.a), t2), mul_dbl(sub3(a.x.b, b.x.b), t2),
+                         mul_dbl(sub3(a.y.a, b.y.a), t2), mul_dbl(sub3(a.y.b, b.y.b), t2)};
+  const uint32_t c2[4] = {s.x.a, s.x.b, s.y.a, s.y.b};
+  return qmat_apply(M, e, c2);
+}
+
 __host__ __device__ __forceinline__ bool qis_zero(qm31 p) { return !(p.x.a | p.x.b | p.y.a | p.y.b); }
 __host__ __device__ __forceinline__ bool qeq(qm31 p, qm31 q) {
   return p.x.a == q.x.a && p.x.b == q.x.b && p.y.a == q.y.a && p.y.b == q.y.b;

@@ -47,10 +47,9 @@ __global__ __launch_bounds__(256) void fold_line_kernel(const uint32_t* __restri
   src = jsoa(src, jo);
   dst = jsoa(dst, jo);
   if (alpha_dev) alpha = *jptr(alpha_dev, jo);
-  uint32_t xi = itw[i];
-  qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
-  qm31 f0 = qadd(a, b), f1 = qscale(qsub(a, b), xi);
-  st(dst, i, qadd(f0, qmul(alpha, f1)));
+  const uint32_t xi = itw[i];
+  const qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
+  st(dst, i, fold_mat(a, b, xi, qmat_of(alpha)));
 }
 
 // accumulate = false: dst is overwritten with the fold (as if it held zeros)
@@ -77,11 +76,9 @@ __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __rest
                                        brev(2 * i, log_n));
     yi = inv(point_at_index(idx).y);
   }
-  qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
-  qm31 f0 = qadd(a, b), f1 = qscale(qsub(a, b), yi);
-  qm31 r = qadd(f0, qmul(alpha, f1));
-  if (accumulate) r = qadd(qmul(ld(dst, i), qsqr(alpha)), r);
-  st(dst, i, r);
+  const qm31 a = ld(src, 2 * (size_t)i), b = ld(src, 2 * (size_t)i + 1);
+  const qmat M = qmat_of(alpha);
+  st(dst, i, accumulate ? fold_mat_acc(a, b, yi, M, ld(dst, i), qmat_of(qsqr(alpha))) : fold_mat(a, b, yi, M));
 }
 
 static Soa4 soa(uint32_t* const* p) { return Soa4{{p[0], p[1], p[2], p[3]}}; }
@@ -93,7 +90,7 @@ static Soa4 soa(uint32_t* const* p) { return Soa4{{p[0], p[1], p[2], p[3]}}; }
 // circle evaluation folded to 2^4 values takes two launches instead of eighteen.
 constexpr uint32_t CHAIN_MAX_S = 13;  // 2^13 inputs per block: 5 x 2^12 words of LDS (80 KiB)
 constexpr int CHAIN_NT = 1024;  // two blocks (32 waves) per CU, 64 VGPRs at most
-constexpr int CHAIN_ROUND = 2;  // first-fold outputs per thread whose loads are in flight together
+constexpr int CHAIN_ROUND = 1;  // first-fold outputs per thread whose loads are in flight together
 constexpr uint32_t CHAIN_SMALL_S = 10;  // tiles of at most 2^10 inputs: 256 threads
 constexpr int CHAIN_SMALL_NT = 256;
 
@@ -105,7 +102,8 @@ struct ChainArgs {
   uint32_t circle;      // 1: the first fold is the circle fold, accumulated into dst[0]
   uint32_t n_fold;      // folds in this launch (<= S)
   uint32_t S;           // each block folds inputs [b 2^S, (b + 1) 2^S)
-  qm31 alpha[CHAIN_MAX_S];
+  qmat mat[CHAIN_MAX_S];  // fold f multiplies by its alpha through this matrix
+  qmat acc_mat;           // the circle fold's alpha^2 (the factor of the layer it accumulates into)
   Soa4 dst[CHAIN_MAX_S];
 };
 
@@ -119,10 +117,6 @@ __device__ __forceinline__ uint32_t circle_twiddle_sign(uint32_t t, uint32_t i) 
   const uint32_t sel = i & 3;
   return (sel == 1 || sel == 2) ? neg(t) : t;
 }
-__device__ __forceinline__ qm31 fold_pair(qm31 a, qm31 b, uint32_t t, qm31 alpha) {
-  return qadd(qadd(a, b), qmul(alpha, qscale(qsub(a, b), t)));
-}
-
 // NT threads for tiles of at most 2^SMAX inputs (the wide launches: 1024 and 2^13; a chain's
 // last layers, at most 2^10 values in one block: 256, which keeps the barriers of its
 // short folds cheap)
@@ -154,7 +148,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void fo
   {
     const uint64_t n_in = 1ull << A.log_in;
     const uint32_t* tw = A.itw + (A.tree_len - (A.circle ? n_in >> 1 : n_in));
-    const qm31 al = A.alpha[0], al2 = qsqr(al);
     const uint32_t g0 = blockIdx.x * h0;
     // CHAIN_ROUND outputs per thread per round: every load of a round is issued before the
     // first store (the stores could alias the loads, so they would otherwise serialise)
@@ -179,8 +172,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void fo
         if (k < h0) {
           const qm31 a = {{w[q][0].x, w[q][1].x}, {w[q][2].x, w[q][3].x}};
           const qm31 b = {{w[q][0].y, w[q][1].y}, {w[q][2].y, w[q][3].y}};
-          qm31 r = fold_pair(a, b, A.circle ? circle_twiddle_sign(t[q], g) : t[q], al);
-          if (A.circle) r = qadd(qmul(d[q], al2), r);
+          const qm31 r = A.circle ? fold_mat_acc(a, b, circle_twiddle_sign(t[q], g), A.mat[0], d[q], A.acc_mat)
+                                  : fold_mat(a, b, t[q], A.mat[0]);
           st(A.dst[0], g, r);
           pl[0][k] = r.x.a;
           pl[1][k] = r.x.b;
@@ -202,7 +195,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void fo
     const uint32_t cnt = h0 >> j;
     const uint32_t* tw = twl + (h0 - 2 * cnt);
     const uint32_t g0 = blockIdx.x * cnt;
-    const qm31 al = A.alpha[j];
+    const qmat& M = A.mat[j];
     qm31 r[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -210,7 +203,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void fo
       if (k < cnt) {
         const qm31 a = {{pl[0][2 * k], pl[1][2 * k]}, {pl[2][2 * k], pl[3][2 * k]}};
         const qm31 b = {{pl[0][2 * k + 1], pl[1][2 * k + 1]}, {pl[2][2 * k + 1], pl[3][2 * k + 1]}};
-        r[q] = fold_pair(a, b, tw[k], al);
+        r[q] = fold_mat(a, b, tw[k], M);
       }
     }
     __syncthreads();  // every read of the previous layer is done
@@ -263,10 +256,11 @@ void fri_fold_chain_async(const uint32_t* itw, uint64_t tree_len, uint32_t log_c
     A.src = fri::soa(cur);
     A.log_in = log_cur;
     A.circle = j == 0;
+    if (A.circle) A.acc_mat = qmat_of(qsqr(alphas[0]));
     A.n_fold = nf;
     A.S = S;
     for (uint32_t f = 0; f < nf; f++) {
-      A.alpha[f] = alphas[j + f];
+      A.mat[f] = qmat_of(alphas[j + f]);
       A.dst[f] = fri::soa(layers + 4 * (size_t)(j + f));
     }
     timed(log_cur > S ? "fri_fold_chain" : "fri_fold_chain_tail", [&] {
