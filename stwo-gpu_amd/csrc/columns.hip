@@ -14,7 +14,9 @@
 namespace sh {
 namespace cols {
 
-// tile = 2^TA x 2^TA elements: 64 x 64 for M31 (256-byte runs), 32 x 32 for QM31 (512-byte runs)
+// tile = 2^TA x 2^TA elements: 64 x 64 for M31 (256-byte runs) below 2^14 values, 32 x 32 for
+// QM31 (512-byte runs; 64 x 64 tiles of 16-byte elements measured slower: 3.1 against 2.5 ms
+// for 2^28)
 template <class T>
 constexpr int tile_bits() {
   return sizeof(T) == 4 ? 6 : 5;
@@ -93,6 +95,55 @@ __global__ __launch_bounds__(BR_NT) void bitrev_tiled16_kernel(T* a, uint32_t lo
   }
 }
 
+// 4-byte elements in columns of at least 2^14: 128 x 128 tiles (512-byte runs; 2^28 values
+// 0.70 -> 0.61 ms against 64 x 64 tiles), both tiles of the pair in registers and one padded
+// LDS tile for the two transposes
+template <class T, int TA, int NT>
+__global__ __launch_bounds__(NT) void bitrev_big_kernel(T* a, uint32_t log_n) {
+  constexpr uint32_t W = 1u << TA, RW = W + 1;
+  constexpr int PER = (int)(W * W / NT);
+  __shared__ T t[W * RW];
+  const uint32_t mid_bits = log_n - 2 * TA;
+  const uint32_t m = blockIdx.x;
+  const uint32_t mr = brev(m, mid_bits);
+  if (mr < m) return;
+  const bool pair = mr != m;
+  auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
+    return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
+  };
+  T va[PER], vb[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    va[k] = a[at(hi, m, lo)];
+    if (pair) vb[k] = a[at(hi, mr, lo)];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    t[hi * RW + lo] = va[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
+    a[at(hi2, mr, lo2)] = t[brev(lo2, TA) * RW + brev(hi2, TA)];
+  }
+  if (!pair) return;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    t[hi * RW + lo] = vb[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
+    a[at(hi2, m, lo2)] = t[brev(lo2, TA) * RW + brev(hi2, TA)];
+  }
+}
+
 template <class T>
 __global__ void bitrev_simple_kernel(T* a, uint32_t n, uint32_t log_n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,9 +163,12 @@ void bit_reverse(T* a, uint64_t n) {
   constexpr int TA = tile_bits<T>();
   if (log_n >= 2 * TA) {
     timed("bit_reverse", [&] {
-      if constexpr (sizeof(T) == 4)
-        bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
-      else
+      if constexpr (sizeof(T) == 4) {
+        if (log_n >= 14)
+          bitrev_big_kernel<T, 7, 1024><<<1u << (log_n - 14), 1024, 0, ctx().cur>>>(a, log_n);
+        else
+          bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
+      } else
         bitrev_tiled16_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
     });
   } else if (n > 1) {
