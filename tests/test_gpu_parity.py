@@ -36,7 +36,9 @@ def rand_qm31(rng):
 
 
 # ------------------------------------------------------------------ column.rs:156-186
-@pytest.mark.parametrize("log_n", [1, 5, 10, 11, 16, 20])
+# sizes either side of each kernel's range: 64 x 64 tiles from 2^12, 128 x 128 tiles from 2^14
+# (2^14 and 2^15: every tile pairs with itself; from 2^16 distinct tiles pair up)
+@pytest.mark.parametrize("log_n", [1, 5, 10, 11, 12, 13, 14, 15, 16, 17, 20, 22])
 def test_bit_reverse_base_field(log_n):
     data = np.arange(1 << log_n, dtype=np.uint32)
     col = S.BaseFieldVec.from_numpy(data)
@@ -44,7 +46,7 @@ def test_bit_reverse_base_field(log_n):
     assert (col.to_numpy() == O.bit_reverse(data)).all()
 
 
-@pytest.mark.parametrize("log_n", [3, 12, 16])
+@pytest.mark.parametrize("log_n", [3, 9, 10, 11, 12, 16])
 def test_bit_reverse_secure_field(log_n):
     data = np.arange(1, 1 + 4 * (1 << log_n), dtype=np.uint32).reshape(-1, 4)
     v = S.SecureFieldVec.from_numpy(data)
