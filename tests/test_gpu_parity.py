@@ -247,6 +247,27 @@ def test_interpolate_columns_out_of_place(logs):
         assert (src[i].to_numpy() == evals[i]).all(), f"source {i} changed"
 
 
+# every multi-column tile size (evaluation domains of 2^3 .. 2^12) at blowups 1..3, whose
+# coefficient count is a template argument of the forward pass, and blowup 4 (the runtime
+# path); 7 columns leave a partial column group in every tile shape
+@pytest.mark.parametrize("log_m,blow", [(m, b) for m in range(3, 13) for b in (1, 2, 3, 4) if m - b >= 1])
+def test_lde_multi_column_tiles(log_m, blow):
+    log_n, n_cols = log_m - blow, 7
+    root = S.canonic_half_coset(log_m)
+    tw, itw = S.cached_twiddles(root)
+    evals = [rand_m31(1 << log_n, 500 * log_m + 10 * blow + c) for c in range(n_cols)]
+    cols = [S.BaseFieldVec.from_numpy(e) for e in evals]
+    ldes = [S.BaseFieldVec.uninitialized(1 << log_m) for _ in range(n_cols)]
+    S.interpolate_columns_async(cols, [log_n] * n_cols, itw, 1 << root.log_size)
+    S.evaluate_columns_async(cols, [log_n] * n_cols, ldes, [log_m] * n_cols, tw, 1 << root.log_size)
+    S.synchronize()
+    otw, oitw = O.precompute_twiddles(root.initial_index, root.step_size, root.log_size)
+    ocoef, olde = O.lde_columns(evals, blow, otw, oitw)
+    for c in range(n_cols):
+        assert (cols[c].to_numpy() == ocoef[c]).all(), f"coefficients of column {c}"
+        assert (ldes[c].to_numpy() == olde[c]).all(), f"evaluations of column {c}"
+
+
 def test_lde_mixed_sizes_one_call():
     logs = [4, 13, 9, 13, 16, 4, 11]
     root = S.canonic_half_coset(17)
