@@ -255,12 +255,24 @@ struct Pass {
         // with the coefficient count known at compile time only the distinct values are
         // loaded: r and r + 2^D read the same coefficient when D = ACT - LO < W
         constexpr int D = ACT < 0 ? W : (ACT - LO <= 0 ? 0 : (ACT - LO < W ? ACT - LO : W));
+        if constexpr (!MASKED && LO == 0 && W == 3) {
+          // 8 consecutive values of a whole column (K >= 3): two 16-byte loads
+          const uint32_t i0 = pb & ((1u << K) - 1);
+          uint4 x0 = {0, 0, 0, 0}, x1 = {0, 0, 0, 0};
+          if (c) {
+            x0 = at(reinterpret_cast<const SH_GLOBAL uint4*>(c), i0 >> 2);
+            x1 = at(reinterpret_cast<const SH_GLOBAL uint4*>(c), (i0 >> 2) + 1);
+          }
+          v[gi * 8 + 0] = x0.x; v[gi * 8 + 1] = x0.y; v[gi * 8 + 2] = x0.z; v[gi * 8 + 3] = x0.w;
+          v[gi * 8 + 4] = x1.x; v[gi * 8 + 5] = x1.y; v[gi * 8 + 6] = x1.z; v[gi * 8 + 7] = x1.w;
+        } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (r < (1 << D))
-            v[gi * R + r] = c ? c[(pb + (r << LO)) & smask] : 0u;
-          else
-            v[gi * R + r] = v[gi * R + (r & ((1 << D) - 1))];
+          for (int r = 0; r < R; r++) {
+            if (r < (1 << D))
+              v[gi * R + r] = c ? c[(pb + (r << LO)) & smask] : 0u;
+            else
+              v[gi * R + r] = v[gi * R + (r & ((1 << D) - 1))];
+          }
         }
       } else if constexpr (LO == 0 && T == 0 && W == 3) {
         const uint32_t i0 = index_of(pb);
@@ -311,8 +323,15 @@ struct Pass {
         const uint32_t col = col_of(pb);
         if (col < A.n_cols) {
           SH_GLOBAL uint32_t* c = as_global(A.dst[col]);
+          if constexpr (LO == 0 && W == 3) {
+            SH_GLOBAL uint4* q = reinterpret_cast<SH_GLOBAL uint4*>(c);
+            const uint32_t qi = (pb & ((1u << K) - 1)) >> 2;
+            at(q, qi) = make_uint4(v[gi * 8 + 0], v[gi * 8 + 1], v[gi * 8 + 2], v[gi * 8 + 3]);
+            at(q, qi + 1) = make_uint4(v[gi * 8 + 4], v[gi * 8 + 5], v[gi * 8 + 6], v[gi * 8 + 7]);
+          } else {
 #pragma unroll
-          for (int r = 0; r < R; r++) c[(pb + (r << LO)) & ((1u << K) - 1)] = v[gi * R + r];
+            for (int r = 0; r < R; r++) c[(pb + (r << LO)) & ((1u << K) - 1)] = v[gi * R + r];
+          }
         }
       } else if constexpr (LO == 0 && T == 0 && W == 3) {
         SH_GLOBAL uint4* q = reinterpret_cast<SH_GLOBAL uint4*>(dst);
