@@ -252,9 +252,6 @@ struct Pass {
       if constexpr (KIND == MULTI) {
         const uint32_t col = col_of(pb);
         const SH_GLOBAL uint32_t* c = col < A.n_cols ? as_global(A.src[col]) : nullptr;
-        // with the coefficient count known at compile time only the distinct values are
-        // loaded: r and r + 2^D read the same coefficient when D = ACT - LO < W
-        constexpr int D = ACT < 0 ? W : (ACT - LO <= 0 ? 0 : (ACT - LO < W ? ACT - LO : W));
         if constexpr (!MASKED && LO == 0 && W == 3) {
           // 8 consecutive values of a whole column (K >= 3): two 16-byte loads
           const uint32_t i0 = pb & ((1u << K) - 1);
@@ -266,6 +263,9 @@ struct Pass {
           v[gi * 8 + 0] = x0.x; v[gi * 8 + 1] = x0.y; v[gi * 8 + 2] = x0.z; v[gi * 8 + 3] = x0.w;
           v[gi * 8 + 4] = x1.x; v[gi * 8 + 5] = x1.y; v[gi * 8 + 6] = x1.z; v[gi * 8 + 7] = x1.w;
         } else {
+          // with the coefficient count known at compile time only the distinct values are
+          // loaded: r and r + 2^D read the same coefficient when D = ACT - LO < W
+          constexpr int D = ACT < 0 ? W : (ACT - LO <= 0 ? 0 : (ACT - LO < W ? ACT - LO : W));
 #pragma unroll
           for (int r = 0; r < R; r++) {
             if (r < (1 << D))
@@ -650,7 +650,6 @@ static const char* pass_name(bool inv, const PassPlan& p) {
 
 template <bool INV, int B, int NT, int K, int KIND>
 static void launch_k(uint32_t blocks, const PassArgs& A) {
-  // forward passes that reach layers at or above act_log (zero-padded coefficients) skip them
   // MULTI tiles evaluated from 2^(K - 1) .. 2^(K - 3) coefficients (blowup 1..3): the
   // coefficient count is a template argument, so the skipped layers cost no branches and only
   // distinct coefficients are loaded (2^16 columns of 2^10 at blowup 4: 0.61 -> 0.57 ms)
@@ -666,6 +665,7 @@ static void launch_k(uint32_t blocks, const PassArgs& A) {
       return;
     }
   }
+  // forward passes that reach layers at or above act_log (zero-padded coefficients) skip them
   if (!INV && A.act_log < A.a + K)
     fft_pass_kernel<false, B, NT, K, KIND, true><<<blocks, NT, 0, ctx().cur>>>(A);
   else
