@@ -55,7 +55,9 @@ def test_bit_reverse_secure_field(log_n):
 
 
 # ------------------------------------------------------------------ field.rs:41-78
-@pytest.mark.parametrize("n", [1, 100, 1 << 12, (1 << 20) + 3])
+# sizes that are multiples of 4 take the 16-byte kernel (12 and 2^16 + 4 leave partial
+# blocks and threads), the others the dword kernel
+@pytest.mark.parametrize("n", [1, 12, 100, 1 << 12, (1 << 16) + 4, (1 << 20) + 3])
 def test_batch_inverse_basefield(n):
     src = np.arange(1, n + 1, dtype=np.uint32)
     a, d = S.BaseFieldVec.from_numpy(src), S.BaseFieldVec.uninitialized(n)
@@ -71,14 +73,19 @@ def test_batch_inverse_secure_field():
     assert (d.to_numpy().reshape(-1) == O.batch_inverse_qm31(src.reshape(-1))).all()
 
 
-def test_batch_inverse_zero_is_isolated():
-    src = np.array([3, 0, 5, 7, 0, 11], np.uint32)
-    a, d = S.BaseFieldVec.from_numpy(src), S.BaseFieldVec.uninitialized(6)
+# zeros lie outside stwo's contract for batch inversion; here inverse(0) is 0 and leaves the
+# other elements' inverses intact, in both the dword (n = 6) and the 16-byte (n = 4096) kernel
+@pytest.mark.parametrize("n", [6, 4096])
+def test_batch_inverse_zero_is_isolated(n):
+    src = np.array([3, 0, 5, 7, 0, 11] * (n // 6) + [13, 0, 17, 19][: n % 6], np.uint32)
+    a, d = S.BaseFieldVec.from_numpy(src), S.BaseFieldVec.uninitialized(n)
     S.batch_inverse(a, d)
     out = d.to_numpy()
-    assert out[1] == 0 and out[4] == 0
-    for i in (0, 2, 3, 5):
-        assert int(out[i]) * int(src[i]) % P == 1
+    for i in range(n):
+        if src[i] == 0:
+            assert out[i] == 0, i
+        else:
+            assert int(out[i]) * int(src[i]) % P == 1, i
 
 
 # ------------------------------------------------------------------ poly.rs:216-249
