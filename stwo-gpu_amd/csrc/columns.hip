@@ -96,51 +96,57 @@ __global__ __launch_bounds__(BR_NT) void bitrev_tiled16_kernel(T* a, uint32_t lo
 }
 
 // 4-byte elements in columns of at least 2^14: 128 x 128 tiles (512-byte runs; 2^28 values
-// 0.70 -> 0.61 ms against 64 x 64 tiles), both tiles of the pair in registers and one padded
-// LDS tile for the two transposes
+// 0.70 -> 0.61 ms against 64 x 64 tiles, 0.58 ms with two elements per 8-byte access), both
+// tiles of the pair in registers and one padded LDS tile for the two transposes
 template <class T, int TA, int NT>
 __global__ __launch_bounds__(NT) void bitrev_big_kernel(T* a, uint32_t log_n) {
   constexpr uint32_t W = 1u << TA, RW = W + 1;
-  constexpr int PER = (int)(W * W / NT);
+  constexpr int PER = (int)(W * W / NT) / 2;  // element pairs per thread and tile
   __shared__ T t[W * RW];
   const uint32_t mid_bits = log_n - 2 * TA;
   const uint32_t m = blockIdx.x;
   const uint32_t mr = brev(m, mid_bits);
   if (mr < m) return;
   const bool pair = mr != m;
-  auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
-    return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
+  // pair e of a tile: row e >> (TA - 1), elements lo, lo + 1 with lo = 2 (e mod W / 2); one
+  // 8-byte access per pair, so a wave covers a whole 512-byte row
+  auto at2 = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
+    return (((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo) >> 1;
   };
-  T va[PER], vb[PER];
+  uint2* const a2 = reinterpret_cast<uint2*>(a);
+  uint2 va[PER], vb[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
-    va[k] = a[at(hi, m, lo)];
-    if (pair) vb[k] = a[at(hi, mr, lo)];
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> (TA - 1), lo = 2 * (e & (W / 2 - 1));
+    va[k] = a2[at2(hi, m, lo)];
+    if (pair) vb[k] = a2[at2(hi, mr, lo)];
   }
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
-    t[hi * RW + lo] = va[k];
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> (TA - 1), lo = 2 * (e & (W / 2 - 1));
+    t[hi * RW + lo] = va[k].x;
+    t[hi * RW + lo + 1] = va[k].y;
   }
   __syncthreads();
+  // element (hi, m, lo) goes to (brev lo, mr, brev hi)
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
-    a[at(hi2, mr, lo2)] = t[brev(lo2, TA) * RW + brev(hi2, TA)];
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> (TA - 1), lo2 = 2 * (e & (W / 2 - 1));
+    a2[at2(hi2, mr, lo2)] = make_uint2(t[brev(lo2, TA) * RW + brev(hi2, TA)], t[brev(lo2 + 1, TA) * RW + brev(hi2, TA)]);
   }
   if (!pair) return;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
-    t[hi * RW + lo] = vb[k];
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> (TA - 1), lo = 2 * (e & (W / 2 - 1));
+    t[hi * RW + lo] = vb[k].x;
+    t[hi * RW + lo + 1] = vb[k].y;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
-    a[at(hi2, m, lo2)] = t[brev(lo2, TA) * RW + brev(hi2, TA)];
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> (TA - 1), lo2 = 2 * (e & (W / 2 - 1));
+    a2[at2(hi2, m, lo2)] = make_uint2(t[brev(lo2, TA) * RW + brev(hi2, TA)], t[brev(lo2 + 1, TA) * RW + brev(hi2, TA)]);
   }
 }
 
