@@ -14,9 +14,8 @@
 namespace sh {
 namespace cols {
 
-// tile = 2^TA x 2^TA elements: 64 x 64 for M31 (256-byte runs) below 2^14 values, 32 x 32 for
-// QM31 (512-byte runs; 64 x 64 tiles of 16-byte elements measured slower: 3.1 against 2.5 ms
-// for 2^28)
+// tile = 2^TA x 2^TA elements for the smaller columns: 64 x 64 for M31 (256-byte runs) below
+// 2^14 values, 32 x 32 for QM31 (512-byte runs) below 2^12 values
 template <class T>
 constexpr int tile_bits() {
   return sizeof(T) == 4 ? 6 : 5;
@@ -150,6 +149,63 @@ __global__ __launch_bounds__(NT) void bitrev_big_kernel(T* a, uint32_t log_n) {
   }
 }
 
+// 16-byte elements in columns of at least 2^12: 64 x 64 tiles (1 KiB runs), both tiles of the
+// pair in registers; LDS row r starts at element 64 r + r / 8, so the transposed reads of 8
+// consecutive lanes (rows 8 apart) fall in distinct bank groups (with one element of padding
+// per row they all hit one group: 3.1 ms for 2^28, against 2.5 ms for 32 x 32 tiles and 2.17
+// ms here)
+__global__ __launch_bounds__(1024) void bitrev_big16_kernel(uint4* a, uint32_t log_n) {
+  constexpr uint32_t TA = 6, W = 1u << TA, NT = 1024;
+  constexpr int PER = (int)(W * W / NT);
+  __shared__ uint4 t[W * W + W / 8];
+  const uint32_t mid_bits = log_n - 2 * TA;
+  const uint32_t m = blockIdx.x;
+  const uint32_t mr = brev(m, mid_bits);
+  if (mr < m) return;
+  const bool pair = mr != m;
+  auto at = [&](uint32_t hi, uint32_t mid, uint32_t lo) -> size_t {
+    return ((size_t)hi << (log_n - TA)) | ((size_t)mid << TA) | lo;
+  };
+  auto row = [](uint32_t r) { return r * W + (r >> 3); };
+  // plain words in registers (arrays of the vector type would be kept in scratch memory)
+  uint32_t va[PER][4], vb[PER][4];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    const uint4 x = a[at(hi, m, lo)];
+    va[k][0] = x.x; va[k][1] = x.y; va[k][2] = x.z; va[k][3] = x.w;
+    if (pair) {
+      const uint4 y = a[at(hi, mr, lo)];
+      vb[k][0] = y.x; vb[k][1] = y.y; vb[k][2] = y.z; vb[k][3] = y.w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    t[row(hi) + lo] = make_uint4(va[k][0], va[k][1], va[k][2], va[k][3]);
+  }
+  __syncthreads();
+  // element (hi, m, lo) goes to (brev lo, mr, brev hi)
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
+    a[at(hi2, mr, lo2)] = t[row(brev(lo2, TA)) + brev(hi2, TA)];
+  }
+  if (!pair) return;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi = e >> TA, lo = e & (W - 1);
+    t[row(hi) + lo] = make_uint4(vb[k][0], vb[k][1], vb[k][2], vb[k][3]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const uint32_t e = threadIdx.x + NT * k, hi2 = e >> TA, lo2 = e & (W - 1);
+    a[at(hi2, m, lo2)] = t[row(brev(lo2, TA)) + brev(hi2, TA)];
+  }
+}
+
 template <class T>
 __global__ void bitrev_simple_kernel(T* a, uint32_t n, uint32_t log_n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -174,7 +230,9 @@ void bit_reverse(T* a, uint64_t n) {
           bitrev_big_kernel<T, 7, 1024><<<1u << (log_n - 14), 1024, 0, ctx().cur>>>(a, log_n);
         else
           bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
-      } else
+      } else if (log_n >= 12)
+        bitrev_big16_kernel<<<1u << (log_n - 12), 1024, 0, ctx().cur>>>(a, log_n);
+      else
         bitrev_tiled16_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
     });
   } else if (n > 1) {

@@ -46,7 +46,8 @@ def test_bit_reverse_base_field(log_n):
     assert (col.to_numpy() == O.bit_reverse(data)).all()
 
 
-@pytest.mark.parametrize("log_n", [3, 9, 10, 11, 12, 16])
+# 32 x 32 tiles from 2^10, 64 x 64 tiles from 2^12 (2^12, 2^13: every tile pairs with itself)
+@pytest.mark.parametrize("log_n", [3, 9, 10, 11, 12, 13, 14, 16])
 def test_bit_reverse_secure_field(log_n):
     data = np.arange(1, 1 + 4 * (1 << log_n), dtype=np.uint32).reshape(-1, 4)
     v = S.SecureFieldVec.from_numpy(data)
