@@ -145,7 +145,7 @@ __device__ __forceinline__ qm31 finish_batch(const Batch& B, const uint64_t (&s)
   return qadd(qmul(acc, B.coeff), qmul_cm(num, dinv));
 }
 
-constexpr int QG = 8;  // columns per load group (a multiple of the fold period 4)
+constexpr int QG = 16;  // columns per load group (a multiple of the fold period 4; 8: 6 % slower at 200 x 2^21)
 
 // js: byte stride between the jobs of a batched launch (blockIdx.z = job): the slot and batch
 // tables and the outputs move by the job's offset (the slots already hold that job's columns)
