@@ -226,7 +226,7 @@ void bit_reverse(T* a, uint64_t n) {
   if (log_n >= 2 * TA) {
     timed("bit_reverse", [&] {
       if constexpr (sizeof(T) == 4) {
-        if (log_n >= 14)
+        if (log_n >= 14 && ((uintptr_t)a & 7) == 0)  // 8-byte accesses
           bitrev_big_kernel<T, 7, 1024><<<1u << (log_n - 14), 1024, 0, ctx().cur>>>(a, log_n);
         else
           bitrev_tiled_kernel<T><<<1u << (log_n - 2 * TA), cols::BR_NT, 0, ctx().cur>>>(a, log_n);
