@@ -4,6 +4,9 @@
 //   facc  a0..a3  b0..b3  t  al0..al3  acc0..acc3                   -> fold_mat_acc (alpha^2 from qsqr)
 //   r64   hi lo                                                     -> reduce64(hi 2^32 + lo)
 //   sub3  a b                                                       -> sub3
+//   mdbl  x t                                                       -> mul_dbl(x, 2t) (= x t)
+//   r62   hi lo / r63 hi lo                                         -> reduce62 / reduce63
+//   qmul  a0..a3  b0..b3                                            -> qmul
 #include <cstdio>
 #include <cstring>
 
@@ -36,6 +39,18 @@ int main() {
       unsigned long long hi, lo;
       if (scanf("%llu %llu", &hi, &lo) != 2) return 1;
       printf("%u\n", reduce64((hi << 32) | lo));
+    } else if (!strcmp(cmd, "mdbl")) {
+      unsigned x, t;
+      if (scanf("%u %u", &x, &t) != 2) return 1;
+      printf("%u\n", mul_dbl(x, t + t));
+    } else if (!strcmp(cmd, "r62") || !strcmp(cmd, "r63")) {
+      unsigned long long hi, lo;
+      if (scanf("%llu %llu", &hi, &lo) != 2) return 1;
+      const unsigned long long y = (hi << 32) | lo;
+      printf("%u\n", cmd[2] == '2' ? reduce62(y) : reduce63(y));
+    } else if (!strcmp(cmd, "qmul")) {
+      const qm31 a = rq(), b = rq();
+      pq(qmul(a, b));
     } else if (!strcmp(cmd, "sub3")) {
       unsigned a, b;
       if (scanf("%u %u", &a, &b) != 2) return 1;

@@ -1,5 +1,6 @@
-"""The device field arithmetic of the FRI folds (stwo-gpu_amd/csrc/field.hpp: qmat_of,
-fold_mat, fold_mat_acc, reduce64, sub3) built for the host and checked against the pure-Python
+"""The device field arithmetic (stwo-gpu_amd/csrc/field.hpp: the FFT butterflies' mul_dbl
+with its doubled twiddle, reduce62/63, qmul, and the FRI folds' qmat_of, fold_mat,
+fold_mat_acc, reduce64, sub3) built for the host and checked against the pure-Python
 field model in refmath.py, at the extreme values where its 64-bit sums are tightest (every
 coordinate P - 1) and at random values.
 
@@ -86,3 +87,32 @@ def test_reduce64_and_sub3(field_bin):
     got = run(field_bin, [f"sub3 {a} {b}" for a, b in pairs])
     for (a, b), line in zip(pairs, got):
         assert int(line) == (a - b) % P, (a, b)
+
+
+def test_mul_dbl_and_reductions(field_bin):
+    # mul_dbl(x, 2t) = x t for x < P, t <= P - 1 (the butterflies' product; its bound hi + lo/2 < 2P
+    # is tightest at x = t = P - 1); reduce62 for products, reduce63 for sums of two products
+    rng = random.Random(4)
+    pairs = list(itertools.product(EDGE, EDGE)) + [(rng.randrange(P), rng.randrange(P)) for _ in range(2000)]
+    got = run(field_bin, [f"mdbl {x} {t}" for x, t in pairs])
+    for (x, t), line in zip(pairs, got):
+        assert int(line) == x * t % P, (x, t)
+    ys62 = [(P - 1) ** 2, 0, 1, P * P - 1] + [rng.randrange(P) * rng.randrange(P) for _ in range(500)]
+    ys62 = [y for y in ys62 if y < 1 << 62]
+    got = run(field_bin, [f"r62 {y >> 32} {y & 0xFFFFFFFF}" for y in ys62])
+    for y, line in zip(ys62, got):
+        assert int(line) == y % P, y
+    ys63 = [2 * (P - 1) ** 2, (1 << 63) - 1] + [rng.randrange(1 << 63) for _ in range(500)]
+    got = run(field_bin, [f"r63 {y >> 32} {y & 0xFFFFFFFF}" for y in ys63])
+    for y, line in zip(ys63, got):
+        assert int(line) == y % P, y
+
+
+def test_qmul(field_bin):
+    rng = random.Random(5)
+    edge_q = [(P - 1,) * 4, (0,) * 4, (1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0), (0, 0, 0, 1), (P - 1, 0, P - 1, 0)]
+    cs = list(itertools.product(edge_q, edge_q)) + [
+        (tuple(rng.randrange(P) for _ in range(4)), tuple(rng.randrange(P) for _ in range(4))) for _ in range(1000)]
+    got = run(field_bin, ["qmul " + " ".join(map(str, [*a, *b])) for a, b in cs])
+    for (a, b), line in zip(cs, got):
+        assert tuple(map(int, line.split())) == R.qm_mul(a, b), (a, b)
